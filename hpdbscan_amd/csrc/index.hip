@@ -1,0 +1,226 @@
+// Occupied-cell table and neighbour ranges on the sorted rows.
+//
+//   k_run_flags        1 where a sorted row starts a new cell
+//   k_compact_cells    cell id and first row of every occupied cell
+//                      (the reference's CellIndex map, spatial_index.h:167-182)
+//   k_neighbor_rows    per occupied cell, the row ranges of its candidate
+//                      cells (spatial_index.h:682-737), in CSR form
+//   k_work_items       (cell, first row) of every 64-row chunk of a cell
+//
+// Neighbour enumeration.  The reference visits every cell whose index differs
+// by at most one in each dimension and lies inside the grid.  Cells that are
+// neighbours along swapped dimension 0 (stride 1) have consecutive ids, so
+// the up-to-three cells (i0-1, i0, i0+1) of one "row" occupy one contiguous
+// range of sorted rows.  A cell therefore has at most 3^(D-1) ranges, found
+// by binary search of the row's first and last cell id in the occupied-cell
+// table.  The row holding the cell itself comes first.
+#include "common.hpp"
+#include "index.hpp"
+#include "internal.hpp"
+
+namespace hpd {
+
+namespace {
+
+inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_run_flags(const K* __restrict__ keys, int64_t n,
+                                                   uint32_t* __restrict__ flags) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_compact_cells(const K* __restrict__ keys, int64_t n,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ scan,
+                                                       K* __restrict__ ucell,
+                                                       int32_t* __restrict__ ustart,
+                                                       const uint32_t* __restrict__ ncells_dev) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n && flags[i]) {
+        const uint32_t u = scan[i];
+        ucell[u] = keys[i];
+        ustart[u] = static_cast<int32_t>(i);
+    }
+    if (i == 0) ustart[*ncells_dev] = static_cast<int32_t>(n);
+}
+
+// first index u in [0, m) with ucell[u] >= key (m if none)
+template <typename K>
+__device__ __forceinline__ int32_t lower_bound_cells(const K* __restrict__ ucell, int32_t m,
+                                                     uint64_t key) {
+    int32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        if (static_cast<uint64_t>(ucell[mid]) < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Decode a cell id into per-swapped-dimension indices.
+__device__ __forceinline__ void decode_cell(uint64_t key, const CellParams& cp, uint64_t* idx) {
+#pragma unroll
+    for (int k = 0; k < kMaxDims; ++k) {
+        if (k < cp.d) {
+            idx[k] = (key / cp.stride[k]) % cp.ncell[k];
+        }
+    }
+}
+
+// Visit the neighbour rows of a cell.  fn(row_lo_key, row_hi_key) gets the
+// inclusive cell-id bounds of each row that lies inside the grid.  Row order:
+// the cell's own row first, then all others in odometer order.
+template <typename F>
+__device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp, F&& fn) {
+    uint64_t idx[kMaxDims];
+    decode_cell(key, cp, idx);
+    const uint64_t lo0 = idx[0] > 0 ? 1 : 0;
+    const uint64_t hi0 = idx[0] + 1 < cp.ncell[0] ? 1 : 0;
+    // the cell's own row
+    fn(key - lo0, key + hi0);
+    int nrows = 1;
+    for (int k = 1; k < cp.d; ++k) nrows *= 3;
+    for (int r = 0; r < nrows; ++r) {
+        // digit k-1 of r in base 3 is the offset (0 -> -1, 1 -> 0, 2 -> +1)
+        int rem = r;
+        bool inside = true;
+        bool center = true;
+        uint64_t base = key;
+        for (int k = 1; k < cp.d; ++k) {
+            const int dgt = rem % 3;
+            rem /= 3;
+            if (dgt == 0) {
+                if (idx[k] == 0) inside = false;
+                base -= cp.stride[k];
+                center = false;
+            } else if (dgt == 2) {
+                if (idx[k] + 1 >= cp.ncell[k]) inside = false;
+                base += cp.stride[k];
+                center = false;
+            }
+        }
+        if (!inside || center) continue;
+        fn(base - lo0, base + hi0);
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict__ ucell,
+                                                             const int32_t* __restrict__ ustart,
+                                                             int32_t m, CellParams cp,
+                                                             uint32_t* __restrict__ nranges,
+                                                             unsigned long long* __restrict__ sumcand) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    uint64_t cand = 0;
+    uint64_t count_u = 0;
+    if (u < m) {
+        uint32_t cnt = 0;
+        for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi) {
+            const int32_t a = lower_bound_cells(ucell, m, lo);
+            const int32_t b = lower_bound_cells(ucell, m, hi + 1);
+            if (b > a) {
+                ++cnt;
+                cand += static_cast<uint64_t>(ustart[b] - ustart[a]);
+            }
+        });
+        nranges[u] = cnt;
+        count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
+    }
+    // Σ over rows of the cell of |candidates| = count_u * cand
+    unsigned long long v = static_cast<unsigned long long>(cand * count_u);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (__lane_id() == 0 && v) atomicAdd(sumcand, v);
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict__ ucell,
+                                                            const int32_t* __restrict__ ustart,
+                                                            int32_t m, CellParams cp,
+                                                            const uint32_t* __restrict__ roff,
+                                                            int2* __restrict__ ranges) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    uint32_t w = roff[u];
+    for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi) {
+        const int32_t a = lower_bound_cells(ucell, m, lo);
+        const int32_t b = lower_bound_cells(ucell, m, hi + 1);
+        if (b > a) ranges[w++] = make_int2(ustart[a], ustart[b]);
+    });
+}
+
+__global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
+                                                      uint32_t* __restrict__ nchunks) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const int32_t c = ustart[u + 1] - ustart[u];
+    nchunks[u] = static_cast<uint32_t>((c + kWave - 1) / kWave);
+}
+
+__global__ __launch_bounds__(256) void k_work_items(const int32_t* __restrict__ ustart, int32_t m,
+                                                    const uint32_t* __restrict__ coff,
+                                                    int2* __restrict__ items) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const int32_t s = ustart[u], e = ustart[u + 1];
+    uint32_t w = coff[u];
+    for (int32_t p = s; p < e; p += kWave) items[w++] = make_int2(u, p);
+}
+
+}  // namespace
+
+template <typename K>
+void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,
+                  int32_t* ustart, uint32_t* ncells_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_run_flags<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, flags);
+    exclusive_scan_u32(flags, flags + n, n, scan_tmp, ncells_dev, s);
+    hipLaunchKernelGGL(k_compact_cells<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n,
+                       static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(flags + n),
+                       ucell, ustart, static_cast<const uint32_t*>(ncells_dev));
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename K>
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+                           uint32_t* nranges, unsigned long long* sumcand, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbor_rows_count<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, ucell,
+                       ustart, m, cp, nranges, sumcand);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename K>
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+                          const uint32_t* roff, int2* ranges, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbor_rows_fill<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, ucell,
+                       ustart, m, cp, roff, ranges);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s) {
+    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+                       nchunks);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_work_items, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m, coff,
+                       items);
+    HPD_HIP(hipGetLastError());
+}
+
+#define HPD_INST_INDEX(K)                                                                        \
+    template void launch_cells<K>(const K*, int64_t, uint32_t*, uint32_t*, K*, int32_t*,         \
+                                  uint32_t*, hipStream_t);                                       \
+    template void launch_neighbor_count<K>(const K*, const int32_t*, int32_t, const CellParams&, \
+                                           uint32_t*, unsigned long long*, hipStream_t);         \
+    template void launch_neighbor_fill<K>(const K*, const int32_t*, int32_t, const CellParams&,  \
+                                          const uint32_t*, int2*, hipStream_t);
+HPD_INST_INDEX(uint32_t)
+HPD_INST_INDEX(uint64_t)
+
+}  // namespace hpd

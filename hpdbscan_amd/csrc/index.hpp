@@ -1,0 +1,33 @@
+// Launchers of the occupied-cell and neighbour-range kernels (index.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace hpd {
+
+// flags must hold 2*n + 1 uint32 (run flags, then their exclusive scan);
+// writes ucell[0..m), ustart[0..m] with ustart[m] = n, and m to *ncells_dev.
+template <typename K>
+void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,
+                  int32_t* ustart, uint32_t* ncells_dev, hipStream_t s);
+
+// nranges[u] = number of non-empty neighbour rows of cell u; adds
+// Σ_u count(u) * |candidates(u)| to *sumcand.
+template <typename K>
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+                           uint32_t* nranges, unsigned long long* sumcand, hipStream_t s);
+
+// ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row
+template <typename K>
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+                          const uint32_t* roff, int2* ranges, hipStream_t s);
+
+void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s);
+void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,
+                       hipStream_t s);
+
+}  // namespace hpd

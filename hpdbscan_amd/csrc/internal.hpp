@@ -1,0 +1,91 @@
+// Host-side declarations shared by the translation units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hpd {
+
+constexpr int kMaxDims = 8;  // dimensions handled by the device kernels
+
+// An error carrying an hpd_status code; the C-ABI layer turns it into the
+// return value and the thread-local message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what);
+#define HPD_HIP(call) ::hpd::check_hip((call), #call)
+
+// Grow-only device arena.  Allocations are 256-byte aligned bump pointers
+// into one hipMalloc'd block; reset() recycles the block for the next call.
+class Arena {
+public:
+    Arena() = default;
+    ~Arena();
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+
+    // Make sure at least `bytes` are available in one block (drops contents).
+    void reserve(size_t bytes);
+    void reset() { used_ = 0; }
+    void* take(size_t bytes);
+    template <typename T>
+    T* take_n(size_t n) {
+        return static_cast<T*>(take(n * sizeof(T)));
+    }
+    size_t capacity() const { return cap_; }
+    size_t used() const { return used_; }
+
+private:
+    char* base_ = nullptr;
+    size_t cap_ = 0;
+    size_t used_ = 0;
+};
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Exclusive prefix sum of n uint32 values (in may equal out).  `tmp` must
+// hold scan_tmp_words(n) uint32.  The total is written to *total_dev when
+// total_dev is not null (device memory).
+size_t scan_tmp_words(int64_t n);
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                        uint32_t* total_dev, hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the
+// key.  Returns true when the result is in (keys_b, vals_b), false when it is
+// in (keys_a, vals_a).  `hist` must hold radix_hist_words(n) uint32 and `tmp`
+// scan_tmp_words(radix_hist_words(n)).
+size_t radix_hist_words(int64_t n);
+template <typename K>
+bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                      int bits, uint32_t* hist, uint32_t* tmp, hipStream_t s);
+
+// Grid geometry of the eps-grid, computed on the host from the min/max of
+// the data exactly as the reference does (spatial_index.h:124-142).
+struct Geometry {
+    int d = 0;
+    uint64_t cells[kMaxDims] = {};   // cells per input dimension
+    int swapped[kMaxDims] = {};      // dimension order, fewest cells first
+    uint64_t stride[kMaxDims] = {};  // stride of swapped dimension k
+    uint64_t ncell[kMaxDims] = {};   // cells of swapped dimension k
+    uint64_t total = 1;              // product of cells (wrapping)
+    bool overflow = false;           // product exceeded 2^64
+    int key_bits = 0;                // bits of the largest cell id
+};
+
+// Device-side parameter blocks (passed by value to kernels).
+struct CellParams {
+    int d;
+    int swapped[kMaxDims];
+    uint64_t stride[kMaxDims];
+    uint64_t ncell[kMaxDims];
+};
+
+}  // namespace hpd

@@ -1,0 +1,137 @@
+/*
+ * C-ABI of the MI355X HPDBSCAN library (libhpdbscan_amd.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an hpd_status;
+ * on failure hpd_last_error() returns a message for the calling thread.
+ *
+ * The reference interface each entry point replaces:
+ *   hpd_cluster            HPDBSCAN::cluster<T>(T* data, int dim0, int dim1,
+ *                          int threads)          include/hpdbscan.h:475-486
+ *                          (and HPDBSCAN::HPDBSCAN(float, size_t) argument
+ *                          checks, include/hpdbscan.h:288-298)
+ *   hpd_cluster_device     the same, with the rows already in device memory
+ *                          and the labels left there
+ *   hpd_context_*          no counterpart: the reference keeps no state
+ *                          between calls; the context owns the device
+ *                          workspace and the stream that replace the OpenMP
+ *                          team of include/hpdbscan.h:364
+ */
+#ifndef HPDBSCAN_CAPI_H
+#define HPDBSCAN_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hpd_status {
+    HPD_OK = 0,
+    HPD_EINVAL = 1,      /* maps to std::invalid_argument */
+    HPD_ENOMEM = 2,      /* device or host allocation failed */
+    HPD_EDEVICE = 3,     /* HIP runtime error or no usable device */
+    HPD_EUNSUPPORTED = 4 /* valid input this build does not handle */
+} hpd_status;
+
+/* Element types of the input rows; the reference instantiates the same ten
+ * (src/hpdbscan.cpp:19-30). */
+typedef enum hpd_dtype {
+    HPD_DTYPE_F32 = 0,
+    HPD_DTYPE_F64 = 1,
+    HPD_DTYPE_I8 = 2,
+    HPD_DTYPE_I16 = 3,
+    HPD_DTYPE_I32 = 4,
+    HPD_DTYPE_I64 = 5,
+    HPD_DTYPE_U8 = 6,
+    HPD_DTYPE_U16 = 7,
+    HPD_DTYPE_U32 = 8,
+    HPD_DTYPE_U64 = 9
+} hpd_dtype;
+
+/* How a border point that lies within eps of cores of two or more clusters
+ * picks its cluster.  The reference (hpdbscan.h:80-91 with rules.h) takes the
+ * cluster of the adjacent core that was processed with the smallest label,
+ * which depends on the OpenMP schedule when threads > 1. */
+typedef enum hpd_border_rule {
+    /* the adjacent cluster with the smallest id; one extra pass, no
+     * dependence on scheduling */
+    HPD_BORDER_MIN_CLUSTER = 0,
+    /* the cluster the reference assigns when run with one thread */
+    HPD_BORDER_ONE_THREAD = 1
+} hpd_border_rule;
+
+typedef struct hpd_context hpd_context;
+
+/* Per-call figures of the last clustering run on a context. */
+typedef struct hpd_run_stats {
+    int64_t n;                   /* rows */
+    int32_t d;                   /* dimensions */
+    int32_t key_bits;            /* significant bits of the cell id */
+    uint64_t total_cells;        /* product of cells per dimension */
+    uint64_t occupied_cells;     /* cells holding at least one row */
+    uint64_t core_points;
+    uint64_t noise_points;
+    uint64_t ambiguous_borders;  /* border rows touching >= 2 clusters */
+    uint64_t sum_candidates;     /* Σ_p |candidate list of p| (the
+                                    reference's distance tests) */
+    /* device time per phase in milliseconds (HIP events on the context's
+       stream; zero unless hpd_context_set_timing(ctx, 1)) */
+    double ms_grid;              /* min/max, cell ids */
+    double ms_sort;              /* radix sort by cell, gather */
+    double ms_index;             /* occupied cells, neighbour ranges */
+    double ms_core;              /* neighbour counts, core flags */
+    double ms_union;             /* core-core union-find */
+    double ms_border;            /* roots, border labels, scatter */
+    double ms_total;             /* first to last event */
+    int32_t core_launches;       /* launches of the core-count kernel */
+    double ms_core_kernel;       /* summed duration of those launches */
+} hpd_run_stats;
+
+/* Thread-local message describing the last failure ("" if none). */
+const char* hpd_last_error(void);
+
+/* Library version string. */
+const char* hpd_version(void);
+
+/* Number of usable HIP devices (0 if none). */
+int hpd_device_count(void);
+
+/* Create a context bound to HIP device `device`.  The context owns a HIP
+ * stream and a grow-only device workspace; it is not thread-safe (use one
+ * per host thread). */
+int hpd_context_create(int device, hpd_context** out);
+int hpd_context_destroy(hpd_context* ctx);
+
+/* Options. */
+int hpd_context_set_border_rule(hpd_context* ctx, int rule);
+int hpd_context_set_timing(hpd_context* ctx, int enabled);
+int hpd_context_get_stats(const hpd_context* ctx, hpd_run_stats* out);
+
+/*
+ * Cluster n rows of d values of type `dtype` (row-major, host memory).
+ * labels_out (host, n entries) receives the reference's labels in input
+ * order: 0 noise, -k core point of cluster k, +k border point of cluster k,
+ * where k - 1 is the smallest sorted position of a core point of the
+ * cluster (include/hpdbscan.h:80, spatial_index.h:743, README.md:54-56).
+ *
+ * Errors: HPD_EINVAL if eps is not > 0 (message "epsilon needs to be
+ * positive", as hpdbscan.h:295-297), if n < 0, d < 1, or a pointer is NULL
+ * while n > 0.  NaN eps is rejected too (the reference accepts it and then
+ * computes garbage grid sizes).
+ */
+int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_t d, float eps,
+                uint64_t min_pts, int64_t* labels_out);
+
+/* Same as hpd_cluster with `data` (n*d values) and `labels_out` (n int64) in
+ * device memory of the context's device.  Work is ordered after all prior
+ * work on `stream` (a hipStream_t, may be NULL for the context's own stream)
+ * and labels_out is complete when the call returns. */
+int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_t d,
+                       float eps, uint64_t min_pts, int64_t* labels_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPDBSCAN_CAPI_H */

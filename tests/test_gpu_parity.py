@@ -1,0 +1,154 @@
+"""Parity of the HIP path (libhpdbscan_amd.so through the C-ABI) with the
+reference, on an MI355X.
+
+Bar: core/noise flags, core cluster ids and one-cluster border labels are
+identical to the reference's one-thread labels; a border row next to cores
+of two clusters carries the id of one of those clusters (the reference's own
+multi-thread behaviour), checked by oracle.check.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, golden_files
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hpdbscan_amd import _lib
+    c = _lib.Context(0)
+    c.set_timing(True)
+    yield c
+    c.close()
+
+
+def assert_parity(x, eps, mp, got, want=None):
+    if want is not None:
+        diff = np.nonzero(got != want)[0]
+        if len(diff) == 0:
+            return
+    mism, bad = oracle.check(x, eps, mp, got)
+    assert mism == 0 and bad == 0, f"{mism} exact mismatches, {bad} bad ambiguous borders"
+
+
+FIXTURES = [f for f in golden_files() if not f.startswith("sumtree_")]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_golden_fixture(ctx, name):
+    z = np.load(os.path.join(GOLDEN, name))
+    x, eps, mp = z["data"], float(z["eps"]), int(z["min_pts"])
+    say(f"fixture {name} n={len(x)} d={x.shape[1]} dtype={x.dtype}")
+    got = ctx.cluster(x, eps, mp)
+    assert_parity(x, eps, mp, got, z["labels"])
+    # the reference's core flags and noise are exact
+    np.testing.assert_array_equal(got < 0, z["labels"] < 0)
+    np.testing.assert_array_equal(got == 0, z["labels"] == 0)
+
+
+@pytest.mark.parametrize("name", golden_files("sumtree_"))
+def test_summation_order_fixture(ctx, name):
+    z = np.load(os.path.join(GOLDEN, name))
+    say(f"sumtree {name}")
+    for x, e, want in zip(z["data"], z["eps"], z["labels"]):
+        np.testing.assert_array_equal(ctx.cluster(x, float(e), 2), want)
+
+
+def blobs(n, d, k, sigma, box, seed, dtype, noise=0.1):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, box, (k, d))
+    nb = int(n * (1 - noise))
+    x = np.concatenate([c[rng.integers(0, k, nb)] + rng.normal(0, sigma, (nb, d)),
+                        rng.uniform(0, box, (n - nb, d))])
+    return x[rng.permutation(n)].astype(dtype)
+
+
+CASES = [(d, dt, seed) for d in range(1, 9) for dt in (np.float32, np.float64) for seed in (0, 1)]
+
+
+@pytest.mark.parametrize("d,dtype,seed", CASES)
+def test_random_blobs_vs_oracle(ctx, d, dtype, seed):
+    rng = np.random.default_rng(seed * 100 + d)
+    n = int(rng.integers(2000, 12000)) if d <= 4 else int(rng.integers(1000, 4000))
+    x = blobs(n, d, int(rng.integers(3, 30)), float(rng.uniform(0.1, 0.4)), 3.0 + d, seed + 7 * d, dtype)
+    eps = float(rng.uniform(0.15, 0.5)) * (1.0 if d <= 4 else 2.0)
+    mp = int(rng.integers(1, 20))
+    say(f"random d={d} dtype={np.dtype(dtype).name} seed={seed} n={n} eps={eps:.3f} mp={mp}")
+    want = oracle.cluster(x, eps, mp)
+    got = ctx.cluster(x, eps, mp)
+    assert_parity(x, eps, mp, got, want)
+    st = ctx.stats()
+    full = oracle.cluster(x, eps, mp, full=True)
+    assert st["core_points"] == full["stats"]["core_points"]
+    assert st["sum_candidates"] == full["stats"]["sum_candidates"]
+    assert st["occupied_cells"] == full["stats"]["occupied_cells"]
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.int32, np.int64,
+                                   np.uint8, np.uint16, np.uint32, np.uint64])
+def test_integer_dtypes_vs_oracle(ctx, dtype):
+    rng = np.random.default_rng(5)
+    lo, hi = (0, 200) if np.dtype(dtype).kind == "u" else (-100, 100)
+    x = rng.integers(lo, hi, size=(4000, 3)).astype(dtype)
+    for eps, mp in ((0.5, 1), (4.0, 3), (9.5, 12)):
+        say(f"int {np.dtype(dtype).name} eps={eps} mp={mp}")
+        assert_parity(x, eps, mp, ctx.cluster(x, eps, mp), oracle.cluster(x, eps, mp))
+
+
+def test_dropin_python_float64(ctx):
+    """HPDBSCAN(eps, minPts).cluster(ndarray) runs in float64 like the SWIG
+    module and returns a tuple."""
+    import hpdbscan_amd as h
+    x32 = blobs(5000, 3, 12, 0.25, 6.0, 3, np.float32)
+    say("dropin float64 via HPDBSCAN.cluster")
+    res = h.HPDBSCAN(0.3, 5).cluster(x32)
+    assert isinstance(res, tuple) and len(res) == len(x32)
+    x64 = x32.astype(np.float64)
+    assert_parity(x64, 0.3, 5, np.array(res, dtype=np.int64), oracle.cluster(x64, 0.3, 5))
+    # keyword arguments and list input
+    res2 = h.HPDBSCAN(epsilon=0.3, min_points=5).cluster(x32.tolist())
+    assert res2 == res
+
+
+def test_determinism_and_empty(ctx):
+    x = blobs(20000, 3, 30, 0.3, 10.0, 9, np.float32)
+    say("determinism n=20000")
+    a = ctx.cluster(x, 0.3, 6)
+    b = ctx.cluster(x, 0.3, 6)
+    np.testing.assert_array_equal(a, b)
+    assert ctx.cluster(np.zeros((0, 3), np.float32), 0.3, 6).shape == (0,)
+
+
+def test_device_entry_point(ctx):
+    torch = pytest.importorskip("torch")
+    x = blobs(30000, 3, 25, 0.3, 10.0, 4, np.float32)
+    say("device entry point n=30000")
+    xd = torch.from_numpy(x).cuda()
+    out = torch.empty(len(x), dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    ctx.cluster_device(xd.data_ptr(), len(x), 3, np.float32, 0.3, 6, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ctx.cluster(x, 0.3, 6))
+
+
+def test_nonfinite_input_rejected(ctx):
+    x = blobs(100, 2, 2, 0.3, 3.0, 1, np.float32)
+    x[5, 1] = np.nan
+    say("nan input")
+    with pytest.raises(ValueError):
+        ctx.cluster(x, 0.3, 3)
+    x[5, 1] = np.inf
+    with pytest.raises(ValueError):
+        ctx.cluster(x, 0.3, 3)
