@@ -1,0 +1,218 @@
+"""Benchmark: Mpoints/s of the HIP clustering path on the 10M x 3-D workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload blobs_10m]
+                    [--no-cpu-baseline] [--cpu-threads T]
+
+One step = one full clustering of the workload's rows (grid, radix sort,
+neighbour ranges, core counts, union-find, border labels, labels written in
+input order) with the rows already resident in device memory and the labels
+left there (hpd_cluster_device).  The timed region is bracketed by stream
+synchronisation; value = rows * steps / seconds / 1e6.
+
+The JSON line also carries
+  roofline      the dominant kernel's achieved rate from HIP events recorded
+                by the library on its own stream during the timed steps
+  cpu_baseline  the reference (oracle/_ref, compiled from the reference's own
+                headers with its own flags) timed on this host on a bounded
+                sample: the rows of the same workload inside a slab
+                x0 < X, which keeps the density (and so the per-row work) of
+                the full workload
+  parity        GPU labels vs the reference's labels on that sample
+Only one GPU is used; --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from hpdbscan_amd import _lib, datagen  # noqa: E402
+
+METRIC = "Mpoints/s at 10M x 3-D (bit-exact labels); batched jobs/s + p99 vs plain"
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector / matrix, dense
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def slab_sample(x: np.ndarray, target: int) -> np.ndarray:
+    """Rows with x[:, 0] below the quantile that keeps about `target` rows."""
+    if target >= len(x):
+        return x
+    cut = np.partition(x[:, 0], target)[target]
+    return np.ascontiguousarray(x[x[:, 0] < cut])
+
+
+def run_reference(sample: np.ndarray, eps: float, mp: int, threads: int, reps: int):
+    exe = os.path.join(ROOT, "oracle", "_ref", "hpdbscan_ref_run")
+    if not os.path.exists(exe):
+        return None, None, "oracle/_ref/hpdbscan_ref_run not built"
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        inp = os.path.join(td, "in.bin")
+        out = os.path.join(td, "labels.bin")
+        sample.astype(np.float32).tofile(inp)
+        cmd = [exe, "f32", inp, str(len(sample)), str(sample.shape[1]), repr(float(eps)), str(mp),
+               str(threads), str(reps), out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            return None, None, r.stderr.strip()[-300:]
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        labels = np.fromfile(out, dtype=np.int64)
+    return res, labels, None
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="blobs_10m", choices=sorted(datagen.WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override the row count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=500_000,
+                    help="rows in the CPU-baseline slab")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build has no "
+              "multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    w = datagen.WORKLOADS[a.workload]
+    t0 = time.time()
+    x, eps, mp = datagen.make(a.workload, a.n or None)
+    n, d = x.shape
+    print(f"[bench] workload {a.workload}: n={n} d={d} eps={eps} minPts={mp} "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
+    ctx = _lib.Context(0)
+    xd = _lib.DeviceBuffer.from_array(ctx, x)
+    ld = _lib.DeviceBuffer(ctx, 8 * n)
+
+    def step():
+        ctx.cluster_device(xd.ptr, n, d, np.float32, eps, mp, ld.ptr)
+
+    ctx.set_timing(False)
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = n * a.steps / elapsed / 1e6
+    print(f"[bench] {a.steps} steps in {elapsed:.3f}s: {ms_per_step:.2f} ms/step, "
+          f"{value:.1f} Mpoints/s", file=sys.stderr, flush=True)
+
+    # phase times (HIP events on the library's stream), separate timed steps
+    ctx.set_timing(True)
+    phases = []
+    for _ in range(max(3, min(a.steps, 5))):
+        step()
+        phases.append(ctx.stats())
+    ctx.set_timing(False)
+    st = phases[-1]
+    keys = ["ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union", "ms_border", "ms_total"]
+    med = {k: float(np.median([p[k] for p in phases])) for k in keys}
+    labels = ld.to_array(np.int64, n)
+
+    # dominant kernel: the neighbour-pair kernels are VALU-bound; the
+    # reference's work is Σcand distance tests of 3D flops each (sub, mul, add
+    # per coordinate: D + D + (D - 1) + 1 compare)
+    dom = max(("ms_core", "ms_union", "ms_border"), key=lambda k: med[k])
+    kname = {"ms_core": "k_core", "ms_union": "k_union", "ms_border": "k_border"}[dom]
+    flops = float(st["sum_candidates"]) * 3 * d
+    achieved = flops / (med[dom] * 1e-3) / 1e12 if med[dom] > 0 else 0.0
+    roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "work": "sum_candidates * 3 * D flops (the reference's distance tests)",
+                "kernel_ms": round(med[dom], 3)}
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "n": n, "d": d, "eps": eps, "min_points": mp,
+                   "seed": w["seed"], "input": "device-resident rows, labels left on device"},
+        "roofline": roofline,
+        "phases_ms": {k: round(v, 3) for k, v in med.items()},
+        "stats": {k: int(st[k]) for k in ("occupied_cells", "core_points", "noise_points",
+                                          "ambiguous_borders", "sum_candidates")},
+    }
+
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or cpu_threads()
+        sample = slab_sample(x, a.cpu_sample)
+        print(f"[bench] cpu baseline: reference on {len(sample)} rows, {threads} threads",
+              file=sys.stderr, flush=True)
+        res, ref_labels, err = run_reference(sample, eps, mp, threads, a.cpu_reps)
+        if res is None:
+            result["cpu_baseline"] = {"value": None, "error": err}
+        else:
+            cpu_v = len(sample) / res["median_s"] / 1e6
+            result["cpu_baseline"] = {
+                "value": round(cpu_v, 4), "unit": "Mpoints/s", "cores": threads, "kind": "reference",
+                "sample": (f"{len(sample)} rows of the {a.workload} workload with x0 below its "
+                           f"{len(sample) / n:.3f} quantile (same density); median of "
+                           f"{a.cpu_reps} runs of HPDBSCAN::cluster<float>(ptr, n, d, {threads})"),
+                "times_s": res["times_s"], "cpu": cpu_model()}
+            gl = ctx.cluster(sample, eps, mp)
+            result["parity"] = {
+                "sample_rows": int(len(sample)),
+                "core_flags_equal": bool(np.array_equal(gl < 0, ref_labels < 0)),
+                "noise_equal": bool(np.array_equal(gl == 0, ref_labels == 0)),
+                "core_ids_equal": bool(np.array_equal(gl[gl < 0], ref_labels[gl < 0])),
+                "label_mismatches": int((gl != ref_labels).sum()),
+                "note": "reference run with several threads; only border rows next to two "
+                        "clusters may differ",
+            }
+            result["speedup_vs_cpu"] = round(value / cpu_v, 1) if cpu_v > 0 else None
+
+    print(json.dumps(result), flush=True)
+    xd.close()
+    ld.close()
+    ctx.close()
+    del labels
+
+
+if __name__ == "__main__":
+    main()

@@ -76,6 +76,12 @@ SIGNATURES = {
                                    ctypes.c_float, ctypes.c_uint64, _vp]),
     "hpd_cluster_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.c_float, ctypes.c_uint64, _vp, _vp]),
+    "hpd_device_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "hpd_device_free": (ctypes.c_int, [_vp, _vp]),
+    "hpd_memcpy_to_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "hpd_memcpy_to_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "hpd_context_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "hpd_context_synchronize": (ctypes.c_int, [_vp]),
 }
 
 _lib = None
@@ -150,6 +156,14 @@ class Context:
     def set_timing(self, enabled: bool) -> None:
         check(load().hpd_context_set_timing(self._h, 1 if enabled else 0))
 
+    def synchronize(self) -> None:
+        check(load().hpd_context_synchronize(self._h))
+
+    def stream(self) -> int:
+        p = _vp()
+        check(load().hpd_context_stream(self._h, ctypes.byref(p)))
+        return p.value or 0
+
     def stats(self) -> dict:
         st = RunStats()
         check(load().hpd_context_get_stats(self._h, ctypes.byref(st)))
@@ -176,6 +190,41 @@ class Context:
         check(load().hpd_cluster_device(self._h, code, _vp(data_ptr), n, d, ctypes.c_float(eps),
                                         int(min_points), _vp(labels_ptr),
                                         _vp(stream) if stream else None))
+
+
+class DeviceBuffer:
+    """Device memory owned through a context (freed on close / GC)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(load().hpd_device_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(ctx, a.nbytes)
+        check(load().hpd_memcpy_to_device(ctx.handle, _vp(buf.ptr), a.ctypes.data, a.nbytes))
+        return buf
+
+    def to_array(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        check(load().hpd_memcpy_to_host(self.ctx.handle, out.ctypes.data, _vp(self.ptr),
+                                        out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self.ptr:
+            check(load().hpd_device_free(self.ctx.handle, _vp(self.ptr)))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default: dict[int, Context] = {}

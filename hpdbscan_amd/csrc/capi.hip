@@ -358,17 +358,17 @@ static void run_dims(Context& ctx, const T* pts, int64_t n, float eps, uint64_t 
     // min / max (spatial_index.h:99-122)
     const int nb = minmax_partial_blocks(n);
     T* part = A.take_n<T>(static_cast<size_t>(nb) * 2 * D);
-    T* mm = A.take_n<T>(2 * D);
-    launch_minmax<T, D>(pts, n, part, mm, s);
+    T* mm = A.take_n<T>(2 * D + 8);
+    int* nonfinite = reinterpret_cast<int*>(mm + 2 * D);
+    HPD_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
+    launch_minmax<T, D>(pts, n, part, mm, nonfinite, s);
     T hmm[2 * D];
-    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, mm, sizeof(hmm), hipMemcpyDeviceToHost, s));
+    int hbad = 0;
+    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, mm, sizeof(hmm) + sizeof(int), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
     std::memcpy(hmm, ctx.host_scratch, sizeof(hmm));
-    if constexpr (std::is_floating_point<T>::value) {
-        for (int j = 0; j < 2 * D; ++j)
-            if (!std::isfinite(hmm[j]))
-                throw Error(HPD_EINVAL, "input contains NaN or infinite coordinates");
-    }
+    std::memcpy(&hbad, reinterpret_cast<const char*>(ctx.host_scratch) + sizeof(hmm), sizeof(int));
+    if (hbad) throw Error(HPD_EINVAL, "input contains NaN or infinite coordinates");
     float rcp_f;
     double rcp_d;
     const Geometry g = make_geometry<T>(hmm, hmm + D, D, eps, &rcp_f, &rcp_d);
@@ -597,6 +597,58 @@ int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_
         hpd::run_any(c, dtype, din, n, d, eps, min_pts, dout, c.stream);
         HPD_HIP(hipMemcpyAsync(labels_out, dout, out_bytes, hipMemcpyDeviceToHost, c.stream));
         HPD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int hpd_device_alloc(hpd_context* ctx, size_t bytes, void** out) {
+    return guarded([&] {
+        if (!ctx || !out) throw Error(HPD_EINVAL, "NULL argument");
+        *out = nullptr;
+        hpd::DeviceGuard g(ctx->impl.device);
+        HPD_HIP(hipMalloc(out, bytes > 0 ? bytes : 1));
+    });
+}
+
+int hpd_device_free(hpd_context* ctx, void* ptr) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        if (!ptr) return;
+        hpd::DeviceGuard g(ctx->impl.device);
+        HPD_HIP(hipStreamSynchronize(ctx->impl.stream));
+        HPD_HIP(hipFree(ptr));
+    });
+}
+
+int hpd_memcpy_to_device(hpd_context* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (!ctx || (bytes && (!dst || !src))) throw Error(HPD_EINVAL, "NULL argument");
+        hpd::DeviceGuard g(ctx->impl.device);
+        HPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->impl.stream));
+        HPD_HIP(hipStreamSynchronize(ctx->impl.stream));
+    });
+}
+
+int hpd_memcpy_to_host(hpd_context* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (!ctx || (bytes && (!dst || !src))) throw Error(HPD_EINVAL, "NULL argument");
+        hpd::DeviceGuard g(ctx->impl.device);
+        HPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->impl.stream));
+        HPD_HIP(hipStreamSynchronize(ctx->impl.stream));
+    });
+}
+
+int hpd_context_stream(hpd_context* ctx, void** stream_out) {
+    return guarded([&] {
+        if (!ctx || !stream_out) throw Error(HPD_EINVAL, "NULL argument");
+        *stream_out = static_cast<void*>(ctx->impl.stream);
+    });
+}
+
+int hpd_context_synchronize(hpd_context* ctx) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        hpd::DeviceGuard g(ctx->impl.device);
+        HPD_HIP(hipStreamSynchronize(ctx->impl.stream));
     });
 }
 

@@ -43,13 +43,15 @@ __device__ __forceinline__ T ref_max(T m, T c) {
 
 template <typename T, int D>
 __global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restrict__ pts, int64_t n,
-                                                               T* __restrict__ part) {
+                                                               T* __restrict__ part,
+                                                               int* __restrict__ nonfinite) {
     T mn[D], mx[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         mn[j] = min_init<T>();
         mx[j] = max_init<T>();
     }
+    bool bad = false;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kMMThreads;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kMMThreads + threadIdx.x; i < n; i += stride) {
 #pragma unroll
@@ -57,8 +59,12 @@ __global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restri
             const T c = pts[i * D + j];
             mn[j] = ref_min(mn[j], c);
             mx[j] = ref_max(mx[j], c);
+            if constexpr (std::is_floating_point<T>::value) bad |= !isfinite(c);
         }
     }
+    // NaN never wins the comparisons above, so non-finite input is flagged
+    // separately (the host rejects it)
+    if (__any(bad) && __lane_id() == 0) atomicOr(nonfinite, 1);
     __shared__ T smn[D][kMMThreads];
     __shared__ T smx[D][kMMThreads];
 #pragma unroll
@@ -195,9 +201,10 @@ int minmax_partial_blocks(int64_t n) {
 }
 
 template <typename T, int D>
-void launch_minmax(const T* pts, int64_t n, T* part, T* out, hipStream_t s) {
+void launch_minmax(const T* pts, int64_t n, T* part, T* out, int* nonfinite, hipStream_t s) {
     const int nb = minmax_partial_blocks(n);
-    hipLaunchKernelGGL((k_minmax_partial<T, D>), dim3(nb), dim3(kMMThreads), 0, s, pts, n, part);
+    hipLaunchKernelGGL((k_minmax_partial<T, D>), dim3(nb), dim3(kMMThreads), 0, s, pts, n, part,
+                       nonfinite);
     hipLaunchKernelGGL((k_minmax_final<T, D>), dim3(1), dim3(64), 0, s, part, nb, out);
     HPD_HIP(hipGetLastError());
 }
@@ -221,7 +228,7 @@ void launch_gather_soa(const T* pts, int64_t n, const uint32_t* order, T* soa, i
 }
 
 #define HPD_INST_GRID(T, D)                                                                     \
-    template void launch_minmax<T, D>(const T*, int64_t, T*, T*, hipStream_t);                  \
+    template void launch_minmax<T, D>(const T*, int64_t, T*, T*, int*, hipStream_t);                         \
     template void launch_cell_ids<T, D, uint32_t>(const T*, int64_t, const T*, const CellParams&, \
                                                   float, double, uint32_t*, uint32_t*,          \
                                                   hipStream_t);                                 \

@@ -26,9 +26,10 @@ namespace hpd {
 int minmax_partial_blocks(int64_t n);
 
 // minmax: part holds 2*D*minmax_partial_blocks(n) values; out receives
-// min[0..D) then max[0..D).
+// min[0..D) then max[0..D); *nonfinite (zeroed by the caller) becomes 1 if a
+// coordinate is NaN or infinite.
 template <typename T, int D>
-void launch_minmax(const T* pts, int64_t n, T* part, T* out, hipStream_t s);
+void launch_minmax(const T* pts, int64_t n, T* part, T* out, int* nonfinite, hipStream_t s);
 
 template <typename T, int D, typename K>
 void launch_cell_ids(const T* pts, int64_t n, const T* minv, const CellParams& cp, float rcp_f,

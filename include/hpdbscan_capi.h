@@ -130,6 +130,19 @@ int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_
 int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_t d,
                        float eps, uint64_t min_pts, int64_t* labels_out, void* stream);
 
+/* Device-memory helpers for callers of hpd_cluster_device that have no HIP
+ * runtime of their own (no counterpart in the reference).  Allocations are
+ * on the context's device; copies are ordered on the context's stream and
+ * complete before the call returns. */
+int hpd_device_alloc(hpd_context* ctx, size_t bytes, void** out);
+int hpd_device_free(hpd_context* ctx, void* ptr);
+int hpd_memcpy_to_device(hpd_context* ctx, void* dst, const void* src, size_t bytes);
+int hpd_memcpy_to_host(hpd_context* ctx, void* dst, const void* src, size_t bytes);
+/* The context's hipStream_t. */
+int hpd_context_stream(hpd_context* ctx, void** stream_out);
+/* Wait for all work on the context's stream. */
+int hpd_context_synchronize(hpd_context* ctx);
+
 #ifdef __cplusplus
 }
 #endif

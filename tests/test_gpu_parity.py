@@ -132,15 +132,19 @@ def test_determinism_and_empty(ctx):
 
 
 def test_device_entry_point(ctx):
-    torch = pytest.importorskip("torch")
+    from hpdbscan_amd import _lib
     x = blobs(30000, 3, 25, 0.3, 10.0, 4, np.float32)
     say("device entry point n=30000")
-    xd = torch.from_numpy(x).cuda()
-    out = torch.empty(len(x), dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    ctx.cluster_device(xd.data_ptr(), len(x), 3, np.float32, 0.3, 6, out.data_ptr(), stream)
-    torch.cuda.synchronize()
-    np.testing.assert_array_equal(out.cpu().numpy(), ctx.cluster(x, 0.3, 6))
+    xd = _lib.DeviceBuffer.from_array(ctx, x)
+    out = _lib.DeviceBuffer(ctx, 8 * len(x))
+    ctx.cluster_device(xd.ptr, len(x), 3, np.float32, 0.3, 6, out.ptr)
+    got = out.to_array(np.int64, len(x))
+    np.testing.assert_array_equal(got, ctx.cluster(x, 0.3, 6))
+    # explicit stream argument: the context's own stream
+    ctx.cluster_device(xd.ptr, len(x), 3, np.float32, 0.3, 6, out.ptr, ctx.stream())
+    np.testing.assert_array_equal(out.to_array(np.int64, len(x)), got)
+    xd.close()
+    out.close()
 
 
 def test_nonfinite_input_rejected(ctx):
