@@ -55,6 +55,8 @@ class RunStats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("core_launches", ctypes.c_int32),
         ("ms_core_kernel", ctypes.c_double),
+        ("boxes", ctypes.c_uint64),
+        ("box_union", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

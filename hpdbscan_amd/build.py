@@ -27,7 +27,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhpdbscan_amd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["capi.hip", "scan.hip", "radix_sort.hip", "grid.hip", "index.hip", "dbscan.hip"]
+SOURCES = ["capi.hip", "scan.hip", "radix_sort.hip", "grid.hip", "index.hip", "dbscan.hip",
+           "boxes.hip"]
 
 ARCH = os.environ.get("HPD_OFFLOAD_ARCH", "gfx950")
 

@@ -1,0 +1,514 @@
+// Sub-boxes of the eps-grid cells and the box-level union of core points.
+//
+// Every occupied cell is split into k^D sub-boxes along the same scaled
+// coordinate the cell index uses, v = (x - min) * (1/eps):
+//   sub_d = clamp(floor(v_d * k) - k * floor(v_d), 0, k - 1)
+// with k = 2 for D <= 3 and k = 3 for D <= 8, so a box is at most eps/k wide
+// and its diagonal at most 0.94 eps.  Rows are regrouped by (cell, box)
+// inside each cell; the reference's sorted position of every row is kept
+// beside it (the cluster ids are smallest positions, not slots).
+//
+// A box is "tight" when the reference's own squared-distance expression
+// evaluated on the box extents, dist2(max - min), is <= EPS2.  Every
+// rounding step of that expression (subtract, square, the summation tree,
+// the flush of tiny values) is monotone in its inputs and |p_d - q_d| <=
+// max_d - min_d for any two members, so every pair of members of a tight
+// box passes the reference's within-eps test, and they share a cell, so they
+// are each other's candidates.  Hence:
+//   * a tight box with >= minPts rows makes all its rows core;
+//   * all cores of a tight box lie in one cluster.
+// The union of the cores is therefore computed on boxes: two boxes with
+// cores are joined when some core pair across them is within eps (tested
+// with the reference's expression, stopping at the first hit), unless they
+// are already joined.  Box pairs whose bounding boxes are farther apart than
+// eps * (1 + 1e-5) cannot hold such a pair (the float result of the
+// expression is within a few ulps of the exact squared distance) and are
+// skipped.
+#include <climits>
+
+#include "boxes.hpp"
+#include "common.hpp"
+#include "grid.hpp"
+#include "internal.hpp"
+
+namespace hpd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+
+inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+template <typename T>
+__device__ __forceinline__ void scaled(T x, T mn, float rcp_f, double rcp_d, double* out_v);
+
+template <>
+__device__ __forceinline__ void scaled<float>(float x, float mn, float rcp_f, double, double* v) {
+    *v = static_cast<double>((x - mn) * rcp_f);
+}
+template <>
+__device__ __forceinline__ void scaled<double>(double x, double mn, float, double rcp_d, double* v) {
+    *v = (x - mn) * rcp_d;
+}
+
+// ---------------------------------------------------------------------------
+// box keys: (cell index << sub_bits) | sub-box digit
+// ---------------------------------------------------------------------------
+template <typename T, int D, typename K>
+__global__ __launch_bounds__(kThreads) void k_box_keys(const T* __restrict__ pts, int64_t n,
+                                                       const uint32_t* __restrict__ inrow_of_pos,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ scan,
+                                                       const T* __restrict__ minv, float rcp_f,
+                                                       double rcp_d, int k, int sub_bits,
+                                                       K* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+    const int64_t pos = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (pos >= n) return;
+    const int64_t row = inrow_of_pos[pos];
+    uint32_t sub = 0, mul = 1;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double v;
+        scaled<T>(pts[row * D + j], minv[j], rcp_f, rcp_d, &v);
+        // v is the value the cell index floors (held exactly in double)
+        const double c = floor(v);
+        long long s = static_cast<long long>(floor(v * k)) - static_cast<long long>(c) * k;
+        s = s < 0 ? 0 : (s >= k ? k - 1 : s);
+        sub += static_cast<uint32_t>(s) * mul;
+        mul *= static_cast<uint32_t>(k);
+    }
+    const uint64_t u = static_cast<uint64_t>(scan[pos] + flags[pos] - 1u);  // occupied-cell index
+    keys[pos] = static_cast<K>((u << sub_bits) | sub);
+    vals[pos] = static_cast<uint32_t>(pos);
+}
+
+// rows in box order: coordinates (SoA), input row and sorted position
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_gather_boxes(const T* __restrict__ pts, int64_t n,
+                                                           const uint32_t* __restrict__ pos_of_slot,
+                                                           const uint32_t* __restrict__ inrow_of_pos,
+                                                           T* __restrict__ soa, int64_t ld,
+                                                           uint32_t* __restrict__ inrow_of_slot) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t row = inrow_of_pos[pos_of_slot[s]];
+    inrow_of_slot[s] = row;
+#pragma unroll
+    for (int j = 0; j < D; ++j) soa[static_cast<int64_t>(j) * ld + s] = pts[static_cast<int64_t>(row) * D + j];
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_box_flags(const K* __restrict__ keys, int64_t n,
+                                                        uint32_t* __restrict__ flags) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ keys, int64_t n,
+                                                        const uint32_t* __restrict__ flags,
+                                                        const uint32_t* __restrict__ scan,
+                                                        int sub_bits, int32_t* __restrict__ bstart,
+                                                        int32_t* __restrict__ bcell,
+                                                        int32_t* __restrict__ box_of_slot,
+                                                        const uint32_t* __restrict__ nbox_dev) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) {
+        const uint32_t b = scan[i] + flags[i] - 1u;
+        box_of_slot[i] = static_cast<int32_t>(b);
+        if (flags[i]) {
+            bstart[b] = static_cast<int32_t>(i);
+            bcell[b] = static_cast<int32_t>(static_cast<uint64_t>(keys[i]) >> sub_bits);
+        }
+    }
+    if (i == 0) bstart[*nbox_dev] = static_cast<int32_t>(n);
+}
+
+// first box of every cell; cbox[m] = number of boxes
+__global__ __launch_bounds__(kThreads) void k_cell_boxes(const int32_t* __restrict__ bcell,
+                                                         int32_t nbox, int32_t m,
+                                                         int32_t* __restrict__ cbox) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b < nbox && (b == 0 || bcell[b] != bcell[b - 1])) cbox[bcell[b]] = b;
+    if (b == 0) cbox[m] = nbox;
+}
+
+// order-preserving integer images of floating-point values (for atomics)
+__device__ __forceinline__ uint32_t ord_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ unsigned long long ord_key(double f) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(f));
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord_val(unsigned long long k) {
+    return __longlong_as_double(static_cast<long long>((k & 0x8000000000000000ull)
+                                                           ? (k & 0x7fffffffffffffffull)
+                                                           : ~k));
+}
+
+template <typename T>
+using OrdKey = std::conditional_t<sizeof(T) == 8, unsigned long long, uint32_t>;
+
+template <typename U>
+__device__ __forceinline__ U wave_min_u(U v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const U w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+template <typename U>
+__device__ __forceinline__ U wave_max_u(U v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const U w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Per box: bounding box (as ordered keys) and the smallest position of a
+// core member.  One wave per 64-row chunk of a box.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_reduce(const T* __restrict__ soa, int64_t ld,
+                                                         const int2* __restrict__ bitems,
+                                                         int32_t nitems,
+                                                         const int32_t* __restrict__ bstart,
+                                                         const uint32_t* __restrict__ pos_of_slot,
+                                                         const uint8_t* __restrict__ core,
+                                                         OrdKey<T>* __restrict__ bbox,
+                                                         int32_t* __restrict__ bmincore) {
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
+    if (wid >= nitems) return;
+    const int2 it = bitems[wid];
+    const int32_t b = uniform(it.x);
+    const int32_t end = uniform(bstart[b + 1]);
+    const int32_t s = it.y + __lane_id();
+    const bool valid = s < end;
+    using U = OrdKey<T>;
+    if (!core) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const U kv = valid ? ord_key(soa[static_cast<int64_t>(j) * ld + s]) : U(0);
+            const U lo = wave_min_u<U>(valid ? kv : ~U(0));
+            const U hi = wave_max_u<U>(valid ? kv : U(0));
+            if (__lane_id() == 0) {
+                atomicMin(&bbox[(static_cast<int64_t>(b) * D + j) * 2], lo);
+                atomicMax(&bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1], hi);
+            }
+        }
+    } else {
+        const int32_t pc = (valid && core[s]) ? static_cast<int32_t>(pos_of_slot[s]) : INT_MAX;
+        int32_t m = pc;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if (__lane_id() == 0 && m != INT_MAX) atomicMin(&bmincore[b], m);
+    }
+}
+
+// tight flag per box: dist2(extent) <= EPS2 in the reference's order
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restrict__ bbox,
+                                                        int32_t nbox, T eps2,
+                                                        const int32_t* __restrict__ bstart,
+                                                        uint32_t min_pts,
+                                                        uint8_t* __restrict__ btight,
+                                                        unsigned int* __restrict__ nloose) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox) return;
+    T lo[D], hi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        lo[j] = ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2]);
+        hi[j] = ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1]);
+    }
+    const bool tight = within<D>(hi, lo, eps2);
+    const int32_t size = bstart[b + 1] - bstart[b];
+    // bit 0: tight; bit 1: tight and at least minPts rows (all rows core)
+    btight[b] = static_cast<uint8_t>((tight ? 1 : 0) |
+                                     ((tight && static_cast<uint32_t>(size) >= min_pts) ? 2 : 0));
+    if (!tight && size > 1) atomicAdd(nloose, 1u);
+}
+
+__device__ __forceinline__ int32_t find_root(int32_t* parent, int32_t x) { return uf_find(parent, x); }
+
+// Box-level union: one thread per box with cores.  Neighbour boxes come
+// from the cell's neighbour rows (cells [a, b) -> boxes [cbox[a], cbox[b])).
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_union(
+    const T* __restrict__ soa, int64_t ld, int32_t nbox, const int32_t* __restrict__ bstart,
+    const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
+    const uint32_t* __restrict__ roff, const int2* __restrict__ rcells,
+    const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ bmincore,
+    const uint8_t* __restrict__ core, T eps2, double gap2_max, int32_t* bparent) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox || bmincore[b] == INT_MAX) return;
+    double lo[D], hi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        lo[j] = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2]));
+        hi[j] = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1]));
+    }
+    const int32_t u = bcell[b];
+    const int32_t s0 = bstart[b], s1 = bstart[b + 1];
+    int32_t rb = find_root(bparent, b);
+    for (uint32_t r = roff[u]; r < roff[u + 1]; ++r) {
+        const int2 cs = rcells[r];
+        const int32_t e0 = cbox[cs.x], e1 = cbox[cs.y];
+        for (int32_t b2 = max(e0, b + 1); b2 < e1; ++b2) {
+            if (bmincore[b2] == INT_MAX) continue;
+            double gap2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const double lo2 = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b2) * D + j) * 2]));
+                const double hi2 = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b2) * D + j) * 2 + 1]));
+                const double g = fmax(0.0, fmax(lo2 - hi[j], lo[j] - hi2));
+                gap2 += g * g;
+            }
+            if (gap2 > gap2_max) continue;
+            int32_t rb2 = find_root(bparent, b2);
+            if (rb2 == rb) continue;
+            // any within-eps core pair across the two boxes?
+            bool hit = false;
+            const int32_t t0 = bstart[b2], t1 = bstart[b2 + 1];
+            for (int32_t i = s0; i < s1 && !hit; ++i) {
+                if (!core[i]) continue;
+                T pi[D];
+#pragma unroll
+                for (int j = 0; j < D; ++j) pi[j] = soa[static_cast<int64_t>(j) * ld + i];
+                for (int32_t q = t0; q < t1; ++q) {
+                    if (!core[q]) continue;
+                    T pq[D];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) pq[j] = soa[static_cast<int64_t>(j) * ld + q];
+                    if (within<D>(pi, pq, eps2)) {
+                        hit = true;
+                        break;
+                    }
+                }
+            }
+            if (hit) {
+                uf_unite(bparent, rb, rb2);
+                rb = find_root(bparent, b);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_box_compmin(int32_t nbox,
+                                                          const int32_t* __restrict__ bmincore,
+                                                          int32_t* bparent,
+                                                          int32_t* __restrict__ compmin) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox || bmincore[b] == INT_MAX) return;
+    const int32_t r = uf_find(bparent, b);
+    atomicMin(&compmin[r], bmincore[b]);
+}
+
+// cluster id (smallest core position of the component) per box with cores
+__global__ __launch_bounds__(kThreads) void k_box_cluster(int32_t nbox,
+                                                          const int32_t* __restrict__ bmincore,
+                                                          int32_t* bparent,
+                                                          const int32_t* __restrict__ compmin,
+                                                          int32_t* __restrict__ bcluster) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox) return;
+    bcluster[b] = bmincore[b] == INT_MAX ? -1 : compmin[uf_find(bparent, b)];
+}
+
+__global__ __launch_bounds__(kThreads) void k_box_items_count(const int32_t* __restrict__ bstart,
+                                                              int32_t nbox,
+                                                              uint32_t* __restrict__ cnt) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox) return;
+    cnt[b] = static_cast<uint32_t>((bstart[b + 1] - bstart[b] + kWave - 1) / kWave);
+}
+
+__global__ __launch_bounds__(kThreads) void k_box_items_fill(const int32_t* __restrict__ bstart,
+                                                             int32_t nbox,
+                                                             const uint32_t* __restrict__ off,
+                                                             int2* __restrict__ items) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox) return;
+    uint32_t w = off[b];
+    for (int32_t s = bstart[b]; s < bstart[b + 1]; s += kWave) items[w++] = make_int2(b, s);
+}
+
+template <typename U>
+__global__ __launch_bounds__(kThreads) void k_fill_bbox(U* __restrict__ bbox, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) bbox[i] = (i & 1) ? U(0) : ~U(0);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_i32(int32_t* __restrict__ a, int32_t v,
+                                                       int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_iota_i32(int32_t* __restrict__ a, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) a[i] = static_cast<int32_t>(i);
+}
+
+}  // namespace
+
+int box_split(int d) { return d <= 3 ? 2 : 3; }
+
+int box_sub_bits(int d) {
+    const int k = box_split(d);
+    uint64_t v = 1;
+    for (int j = 0; j < d; ++j) v *= static_cast<uint64_t>(k);
+    const uint64_t mx = v - 1;
+    return mx == 0 ? 0 : 64 - __builtin_clzll(mx);
+}
+
+template <typename T, int D, typename K>
+void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
+                     const uint32_t* scan, const T* minv, float rcp_f, double rcp_d, K* keys,
+                     uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL((k_box_keys<T, D, K>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                       pts, n, inrow_of_pos, flags, scan, minv, rcp_f, rcp_d, box_split(D),
+                       box_sub_bits(D), keys, vals);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
+                         const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                       pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename K>
+void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
+                      int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_box_flags<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
+                       flags);
+    exclusive_scan_u32(flags, flags + n, n, scan_tmp, nbox_dev, s);
+    hipLaunchKernelGGL(k_box_table<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
+                       static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(flags + n),
+                       sub_bits, bstart, bcell, box_of_slot, static_cast<const uint32_t*>(nbox_dev));
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_boxes, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, bcell,
+                       nbox, m, cbox);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32_t* scan_tmp,
+                      uint32_t* total_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_items_count, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       bstart, nbox, cnt);
+    exclusive_scan_u32(cnt, cnt, nbox, scan_tmp, total_dev, s);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off, int2* items,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_box_items_fill, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       bstart, nbox, off, items);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bmincore,
+                       hipStream_t s) {
+    using U = OrdKey<T>;
+    if (!core) {
+        const int64_t nb = static_cast<int64_t>(a.nbox) * D * 2;
+        hipLaunchKernelGGL(k_fill_bbox<U>, dim3(blocks_for(nb, kThreads)), dim3(kThreads), 0, s,
+                           reinterpret_cast<U*>(a.bbox), nb);
+    } else {
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                           bmincore, INT_MAX, static_cast<int64_t>(a.nbox));
+    }
+    hipLaunchKernelGGL((k_box_reduce<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart,
+                       a.pos_of_slot, core, reinterpret_cast<U*>(a.bbox), bmincore);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_tight(const BoxArgs<T>& a, T eps2, uint32_t min_pts, uint8_t* btight,
+                      unsigned int* nloose, hipStream_t s) {
+    using U = OrdKey<T>;
+    hipLaunchKernelGGL((k_box_tight<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
+                       s, reinterpret_cast<const U*>(a.bbox), a.nbox, eps2, a.bstart, min_pts,
+                       btight, nloose);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                      const int32_t* bmincore, const uint8_t* core, T eps2, int32_t* bparent,
+                      int32_t* compmin, int32_t* bcluster, hipStream_t s) {
+    using U = OrdKey<T>;
+    const double e2 = static_cast<double>(eps2);
+    const double gap2_max = e2 * (1.0 + 1e-5) + 1e-30;
+    hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                       bparent, static_cast<int64_t>(a.nbox));
+    hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
+                       s, a.soa, a.ld, a.nbox, a.bstart, a.bcell, a.cbox, roff, rcells,
+                       reinterpret_cast<const U*>(a.bbox), bmincore, core, eps2, gap2_max, bparent);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                       compmin, INT_MAX, static_cast<int64_t>(a.nbox));
+    hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                       a.nbox, bmincore, bparent, compmin);
+    hipLaunchKernelGGL(k_box_cluster, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                       a.nbox, bmincore, bparent, static_cast<const int32_t*>(compmin), bcluster);
+    HPD_HIP(hipGetLastError());
+}
+
+#define HPD_INST_BOX_KEYS(T, D)                                                                   \
+    template void launch_box_keys<T, D, uint32_t>(const T*, int64_t, const uint32_t*,             \
+                                                  const uint32_t*, const uint32_t*, const T*,     \
+                                                  float, double, uint32_t*, uint32_t*,            \
+                                                  hipStream_t);                                   \
+    template void launch_box_keys<T, D, uint64_t>(const T*, int64_t, const uint32_t*,             \
+                                                  const uint32_t*, const uint32_t*, const T*,     \
+                                                  float, double, uint64_t*, uint32_t*,            \
+                                                  hipStream_t);                                   \
+    template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, const uint32_t*,  \
+                                            T*, int64_t, uint32_t*, hipStream_t);                 \
+    template void launch_box_reduce<T, D>(const BoxArgs<T>&, const uint8_t*, int32_t*,            \
+                                          hipStream_t);                                           \
+    template void launch_box_tight<T, D>(const BoxArgs<T>&, T, uint32_t, uint8_t*,               \
+                                         unsigned int*, hipStream_t);                             \
+    template void launch_box_union<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,         \
+                                         const int32_t*, const uint8_t*, T, int32_t*, int32_t*,   \
+                                         int32_t*, hipStream_t);
+#define HPD_INST_BOX_ALL_D(T) \
+    HPD_INST_BOX_KEYS(T, 1)   \
+    HPD_INST_BOX_KEYS(T, 2)   \
+    HPD_INST_BOX_KEYS(T, 3)   \
+    HPD_INST_BOX_KEYS(T, 4)   \
+    HPD_INST_BOX_KEYS(T, 5)   \
+    HPD_INST_BOX_KEYS(T, 6)   \
+    HPD_INST_BOX_KEYS(T, 7)   \
+    HPD_INST_BOX_KEYS(T, 8)
+HPD_INST_BOX_ALL_D(float)
+HPD_INST_BOX_ALL_D(double)
+
+template void launch_box_table<uint32_t>(const uint32_t*, int64_t, int, uint32_t*, uint32_t*,
+                                         int32_t*, int32_t*, int32_t*, uint32_t*, hipStream_t);
+template void launch_box_table<uint64_t>(const uint64_t*, int64_t, int, uint32_t*, uint32_t*,
+                                         int32_t*, int32_t*, int32_t*, uint32_t*, hipStream_t);
+
+}  // namespace hpd

@@ -1,0 +1,75 @@
+// Launchers of the sub-box kernels (boxes.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace hpd {
+
+// boxes per cell and dimension (k) and bits of the sub-box digit
+int box_split(int d);
+int box_sub_bits(int d);
+
+template <typename T>
+struct BoxArgs {
+    const T* soa = nullptr;            // coordinates in slot order, [D][ld]
+    int64_t ld = 0;
+    int32_t nbox = 0;
+    const int32_t* bstart = nullptr;   // first slot of every box, [nbox + 1]
+    const int32_t* bcell = nullptr;    // occupied cell of every box
+    const int32_t* cbox = nullptr;     // first box of every cell, [m + 1]
+    const int2* bitems = nullptr;      // (box, first slot) per 64-slot chunk
+    int32_t nbitems = 0;
+    const uint32_t* pos_of_slot = nullptr;
+    void* bbox = nullptr;              // ordered-key min/max per box and dim
+};
+
+// key = (occupied-cell index << sub_bits) | sub-box digit, val = position
+template <typename T, int D, typename K>
+void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
+                     const uint32_t* scan, const T* minv, float rcp_f, double rcp_d, K* keys,
+                     uint32_t* vals, hipStream_t s);
+
+template <typename T, int D>
+void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
+                         const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
+                         hipStream_t s);
+
+// flags: 2n + 2 words; bstart: nbox + 1; writes *nbox_dev
+template <typename K>
+void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
+                      int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
+                      hipStream_t s);
+
+void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
+                       hipStream_t s);
+
+// cnt[b] = 64-slot chunks of box b, scanned in place; *total_dev = chunks
+void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32_t* scan_tmp,
+                      uint32_t* total_dev, hipStream_t s);
+void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off, int2* items,
+                           hipStream_t s);
+
+// core == nullptr: bounding boxes; otherwise the smallest core position of
+// every box into bmincore (INT_MAX for boxes without cores)
+template <typename T, int D>
+void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bmincore,
+                       hipStream_t s);
+
+// btight[b]: bit 0 tight, bit 1 tight with >= minPts rows; *nloose counts
+// boxes of two or more rows that are not tight
+template <typename T, int D>
+void launch_box_tight(const BoxArgs<T>& a, T eps2, uint32_t min_pts, uint8_t* btight,
+                      unsigned int* nloose, hipStream_t s);
+
+// components of the boxes with cores; bcluster[b] = smallest core position
+// of b's component (-1 without cores)
+template <typename T, int D>
+void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                      const int32_t* bmincore, const uint8_t* core, T eps2, int32_t* bparent,
+                      int32_t* compmin, int32_t* bcluster, hipStream_t s);
+
+}  // namespace hpd

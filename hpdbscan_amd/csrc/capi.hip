@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/hpdbscan_capi.h"
+#include "boxes.hpp"
 #include "dbscan.hpp"
 #include "grid.hpp"
 #include "index.hpp"
@@ -233,32 +234,63 @@ private:
 // --------------------------------------------------------------------------
 // one clustering run
 // --------------------------------------------------------------------------
+static inline int bits_for(uint64_t max_value) {
+    return max_value == 0 ? 0 : 64 - __builtin_clzll(max_value);
+}
+
+// worst-case device workspace of one run with n rows
+template <typename T, int D>
+static size_t workspace_bytes(int64_t n) {
+    const size_t nn = static_cast<size_t>(n);
+    const size_t L = align_up(nn, 16);
+    const size_t hist_words = radix_hist_words(n);
+    const size_t tmp_words = scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2)));
+    size_t b = 0;
+    b += 4 * 8 * nn + 4 * 4 * nn;              // keys / values of both sorts
+    b += 4 * hist_words + 4 * tmp_words;       // radix histogram, scan scratch
+    b += sizeof(T) * L * D;                    // coordinates in slot order
+    b += 4 * (2 * nn + 2);                     // run flags + scan
+    b += 8 * nn + 3 * 4 * (nn + 1);            // ucell, ustart, nranges, nchunks
+    b += L + 3 * 4 * L + 3 * 4 * nn;           // core, parent, rsnap, rootslot, cmin/cmax/rcenter
+    b += 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // box tables
+    b += 2 * D * 8 * nn;                       // box bounding boxes
+    b += sizeof(T) * 2 * D * 1024 + 40 * 256 + 64 * 1024;  // min/max partials, alignment
+    return b;
+}
+
 template <typename T, int D, typename K>
 static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t min_pts,
                       int64_t* labels, hipStream_t s, const Geometry& g, float rcp_f,
-                      double rcp_d, const T* minv_dev, PhaseTimer& tm, int64_t* stage) {
+                      double rcp_d, const T* minv_dev, PhaseTimer& tm) {
+    constexpr bool kFloat = std::is_floating_point<T>::value;
     Arena& A = ctx.main;
-    K* keys_a = A.take_n<K>(static_cast<size_t>(n));
-    K* keys_b = A.take_n<K>(static_cast<size_t>(n));
-    uint32_t* vals_a = A.take_n<uint32_t>(static_cast<size_t>(n));
-    uint32_t* vals_b = A.take_n<uint32_t>(static_cast<size_t>(n));
+    const size_t nn = static_cast<size_t>(n);
+    const int64_t ld = (n + 15) & ~int64_t(15);  // rows are read in aligned batches of <= 16
+    const size_t L = static_cast<size_t>(ld);
+    K* keys_a = A.take_n<K>(nn);
+    K* keys_b = A.take_n<K>(nn);
+    uint32_t* vals_a = A.take_n<uint32_t>(nn);
+    uint32_t* vals_b = A.take_n<uint32_t>(nn);
     const size_t hist_words = radix_hist_words(n);
     uint32_t* hist = A.take_n<uint32_t>(hist_words);
-    const size_t tmp_words = scan_tmp_words(static_cast<int64_t>(
-        std::max<size_t>(hist_words, static_cast<size_t>(2 * n + 2))));
-    uint32_t* tmp = A.take_n<uint32_t>(tmp_words);
-    const int64_t ld = (n + 15) & ~int64_t(15);  // rows are read in aligned batches of <= 16
-    T* soa = A.take_n<T>(static_cast<size_t>(ld) * D);
-    uint32_t* flags = A.take_n<uint32_t>(static_cast<size_t>(2 * n + 2));
-    K* ucell = A.take_n<K>(static_cast<size_t>(n));
-    int32_t* ustart = A.take_n<int32_t>(static_cast<size_t>(n + 1));
-    uint32_t* nranges = A.take_n<uint32_t>(static_cast<size_t>(n + 1));
-    uint32_t* nchunks = A.take_n<uint32_t>(static_cast<size_t>(n + 1));
-    uint8_t* core = A.take_n<uint8_t>(static_cast<size_t>(ld));
-    int32_t* parent = A.take_n<int32_t>(static_cast<size_t>(n));
+    uint32_t* tmp = A.take_n<uint32_t>(scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2))));
+    T* soa = A.take_n<T>(L * D);
+    uint32_t* flags = A.take_n<uint32_t>(2 * nn + 2);
+    K* ucell = A.take_n<K>(nn);
+    int32_t* ustart = A.take_n<int32_t>(nn + 1);
+    uint32_t* nranges = A.take_n<uint32_t>(nn + 1);
+    uint32_t* nchunks = A.take_n<uint32_t>(nn + 1);
+    uint8_t* core = A.take_n<uint8_t>(L);
+    int32_t* parent = A.take_n<int32_t>(L);
+    int32_t* rsnap = A.take_n<int32_t>(L);
+    int32_t* rootslot = A.take_n<int32_t>(L);
+    int32_t* cmin = A.take_n<int32_t>(nn);
+    int32_t* cmax = A.take_n<int32_t>(nn);
+    int32_t* rcenter = A.take_n<int32_t>(nn);
     unsigned long long* dstats = A.take_n<unsigned long long>(16);
-    uint32_t* counters = reinterpret_cast<uint32_t*>(dstats + 8);  // [0]=m [1]=ranges [2]=items
+    uint32_t* counters = reinterpret_cast<uint32_t*>(dstats + 8);  // m, ranges, items, boxes, box items, loose
     HPD_HIP(hipMemsetAsync(dstats, 0, 16 * sizeof(unsigned long long), s));
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.host_scratch);
 
     CellParams cp{};
     cp.d = D;
@@ -271,42 +303,103 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     // cell ids (spatial_index.h:144-165)
     launch_cell_ids<T, D, K>(pts, n, minv_dev, cp, rcp_f, rcp_d, keys_a, vals_a, s);
     tm.mark();  // 1: grid done
-    *stage = 1;
 
-    // stable sort by cell (spatial_index.h:167-216)
+    // stable sort by cell (spatial_index.h:167-216): position -> input row
     const bool in_b = radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, g.key_bits, hist, tmp, s);
     K* keys = in_b ? keys_b : keys_a;
-    uint32_t* order = in_b ? vals_b : vals_a;
-    launch_gather_soa<T, D>(pts, n, order, soa, ld, s);
-    tm.mark();  // 2: sort done
-    *stage = 2;
+    uint32_t* inrow_of_pos = in_b ? vals_b : vals_a;
 
-    // occupied cells and neighbour ranges (spatial_index.h:167-182, 682-737)
+    // occupied cells (spatial_index.h:167-182)
     launch_cells<K>(keys, n, flags, tmp, ucell, ustart, &counters[0], s);
-    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hc, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
-    const int32_t m = static_cast<int32_t>(reinterpret_cast<uint32_t*>(ctx.host_scratch)[0]);
+    const int32_t m = static_cast<int32_t>(hc[0]);
 
+    // slot order: in box mode, rows regrouped by sub-box inside each cell
+    const bool boxes = kFloat;
+    const uint32_t* pos_of_slot = nullptr;
+    uint32_t* inrow_of_slot = inrow_of_pos;
+    int32_t *box_of_slot = nullptr, *bstart = nullptr, *bcell = nullptr, *cbox = nullptr;
+    uint32_t* bcnt = nullptr;
+    int32_t *bmincore = nullptr, *bparent = nullptr, *compmin = nullptr, *bcluster = nullptr;
+    uint8_t* btight = nullptr;
+    void* bbox = nullptr;
+    if constexpr (kFloat) {
+        const int sub_bits = box_sub_bits(D);
+        const int bits2 = bits_for(static_cast<uint64_t>(m - 1)) + sub_bits;
+        uint64_t* k2a = A.take_n<uint64_t>(nn);
+        uint64_t* k2b = A.take_n<uint64_t>(nn);
+        uint32_t* v2a = A.take_n<uint32_t>(nn);
+        uint32_t* v2b = A.take_n<uint32_t>(nn);
+        inrow_of_slot = A.take_n<uint32_t>(nn);
+        box_of_slot = A.take_n<int32_t>(nn);
+        bstart = A.take_n<int32_t>(nn + 1);
+        bcell = A.take_n<int32_t>(nn);
+        cbox = A.take_n<int32_t>(nn + 1);
+        bcnt = A.take_n<uint32_t>(nn + 1);
+        bmincore = A.take_n<int32_t>(nn);
+        bparent = A.take_n<int32_t>(nn);
+        compmin = A.take_n<int32_t>(nn);
+        bcluster = A.take_n<int32_t>(nn);
+        btight = A.take_n<uint8_t>(nn);
+        bbox = A.take(2 * D * sizeof(T) * nn);
+        const uint32_t* slot_pos;
+        const void* sorted2;
+        if (bits2 <= 32) {
+            uint32_t* ka = reinterpret_cast<uint32_t*>(k2a);
+            uint32_t* kb = reinterpret_cast<uint32_t*>(k2b);
+            launch_box_keys<T, D, uint32_t>(pts, n, inrow_of_pos, flags, flags + n, minv_dev, rcp_f,
+                                            rcp_d, ka, v2a, s);
+            const bool ib = radix_sort_pairs<uint32_t>(ka, v2a, kb, v2b, n, bits2, hist, tmp, s);
+            slot_pos = ib ? v2b : v2a;
+            sorted2 = ib ? static_cast<const void*>(kb) : static_cast<const void*>(ka);
+            launch_box_table<uint32_t>(static_cast<const uint32_t*>(sorted2), n, sub_bits, flags, tmp,
+                                       bstart, bcell, box_of_slot, &counters[3], s);
+        } else {
+            launch_box_keys<T, D, uint64_t>(pts, n, inrow_of_pos, flags, flags + n, minv_dev, rcp_f,
+                                            rcp_d, k2a, v2a, s);
+            const bool ib = radix_sort_pairs<uint64_t>(k2a, v2a, k2b, v2b, n, bits2, hist, tmp, s);
+            slot_pos = ib ? v2b : v2a;
+            sorted2 = ib ? static_cast<const void*>(k2b) : static_cast<const void*>(k2a);
+            launch_box_table<uint64_t>(static_cast<const uint64_t*>(sorted2), n, sub_bits, flags, tmp,
+                                       bstart, bcell, box_of_slot, &counters[3], s);
+        }
+        pos_of_slot = slot_pos;
+        launch_gather_boxes<T, D>(pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot, s);
+    } else {
+        launch_gather_soa<T, D>(pts, n, inrow_of_pos, soa, ld, s);
+    }
+    tm.mark();  // 2: sort done
+
+    // neighbour ranges (spatial_index.h:682-737) and work items
     launch_neighbor_count<K>(ucell, ustart, m, cp, nranges, &dstats[0], s);
     exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
     exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
     HPD_HIP(hipMemcpyAsync(&counters[1], &nranges[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(&counters[2], &nchunks[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hc, counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
-    const uint32_t* hc = reinterpret_cast<const uint32_t*>(ctx.host_scratch);
     const uint32_t total_ranges = hc[1];
     const int32_t nitems = static_cast<int32_t>(hc[2]);
+    const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
+    int32_t nbitems = 0;
+    if (boxes) {
+        launch_box_items(bstart, nbox, bcnt, tmp, &counters[4], s);
+        HPD_HIP(hipMemcpyAsync(hc + 4, &counters[4], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HPD_HIP(hipStreamSynchronize(s));
+        nbitems = static_cast<int32_t>(hc[4]);
+    }
 
-    ctx.aux.reserve(align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
-                    align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) + 1024);
+    ctx.aux.reserve(2 * align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) + 1024);
     int2* ranges = ctx.aux.take_n<int2>(total_ranges);
+    int2* rcells = ctx.aux.take_n<int2>(total_ranges);
     int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
-    launch_neighbor_fill<K>(ucell, ustart, m, cp, nranges, ranges, s);
+    int2* bitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
+    launch_neighbor_fill<K>(ucell, ustart, m, cp, nranges, ranges, rcells, rcenter, s);
     launch_work_items(ustart, m, nchunks, items, s);
-    tm.mark();  // 3: index done
-    *stage = 3;
 
     DbscanArgs<T> a;
     a.soa = soa;
@@ -317,18 +410,56 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     a.ustart = ustart;
     a.roff = nranges;
     a.ranges = ranges;
+    a.rcells = rcells;
+    a.rcenter = rcenter;
+    a.m = m;
     a.eps2 = eps2_as<T>(eps);
     a.min_pts = static_cast<uint32_t>(std::min<uint64_t>(min_pts, 0xffffffffull));
+    a.pos_of_slot = pos_of_slot;
+    a.inrow = inrow_of_slot;
+
+    BoxArgs<T> ba;
+    bool box_union = false;
+    if constexpr (kFloat) {
+        launch_cell_boxes(bcell, nbox, m, cbox, s);
+        launch_box_items_fill(bstart, nbox, bcnt, bitems, s);
+        ba.soa = soa;
+        ba.ld = ld;
+        ba.nbox = nbox;
+        ba.bstart = bstart;
+        ba.bcell = bcell;
+        ba.cbox = cbox;
+        ba.bitems = bitems;
+        ba.nbitems = nbitems;
+        ba.pos_of_slot = pos_of_slot;
+        ba.bbox = bbox;
+        launch_box_reduce<T, D>(ba, nullptr, nullptr, s);
+        launch_box_tight<T, D>(ba, a.eps2, a.min_pts, btight, &counters[5], s);
+        a.box_of_slot = box_of_slot;
+        a.btight = btight;
+    }
+    tm.mark();  // 3: index done
 
     launch_core<T, D>(a, core, &dstats[1], s);
     tm.mark();  // 4: core done
-    *stage = 4;
-    launch_union<T, D>(a, core, parent, s);
+
+    if constexpr (kFloat) {
+        HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HPD_HIP(hipStreamSynchronize(s));
+        box_union = hc[5] == 0;  // every box of two or more rows is tight
+    }
+    if (box_union) {
+        launch_box_reduce<T, D>(ba, core, bmincore, s);
+        launch_box_union<T, D>(ba, nranges, rcells, bmincore, core, a.eps2, bparent, compmin,
+                               bcluster, s);
+    } else {
+        if (!compmin) compmin = A.take_n<int32_t>(nn);
+        launch_union_points<T, D>(a, core, parent, rsnap, cmin, cmax, compmin, s);
+    }
     tm.mark();  // 5: union done
-    *stage = 5;
-    launch_labels<T, D>(a, core, parent, order, labels, &dstats[2], s);
+    launch_labels<T, D>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
+                        cmax, labels, &dstats[2], s);
     tm.mark();  // 6: labels done
-    *stage = 6;
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
     HPD_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -338,6 +469,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     ctx.stats.noise_points = hs[2];
     ctx.stats.ambiguous_borders = hs[3];
     ctx.stats.occupied_cells = static_cast<uint64_t>(m);
+    ctx.stats.boxes = static_cast<uint64_t>(nbox);
+    ctx.stats.box_union = box_union ? 1 : 0;
 }
 
 template <typename T, int D>
@@ -346,14 +479,7 @@ static void run_dims(Context& ctx, const T* pts, int64_t n, float eps, uint64_t 
     PhaseTimer tm(ctx, s);
     tm.mark();  // 0: start
     Arena& A = ctx.main;
-    // worst-case workspace for this n (keys are 64-bit in the worst case)
-    const size_t nn = static_cast<size_t>(n);
-    const size_t hist_words = radix_hist_words(n);
-    const size_t bytes = 4 * 8 * nn + 2 * 4 * nn + 4 * hist_words +
-                         4 * scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2))) +
-                         sizeof(T) * (nn + 16) * D + 4 * (2 * nn + 2) + 8 * nn + 3 * 4 * (nn + 1) + nn + 16 +
-                         4 * nn + sizeof(T) * 2 * D * 1024 + 64 * 1024;
-    A.reserve(bytes);
+    A.reserve(workspace_bytes<T, D>(n));
 
     // min / max (spatial_index.h:99-122)
     const int nb = minmax_partial_blocks(n);
@@ -378,13 +504,10 @@ static void run_dims(Context& ctx, const T* pts, int64_t n, float eps, uint64_t 
     ctx.stats.total_cells = g.total;
     ctx.stats.key_bits = g.key_bits;
 
-    int64_t stage = 0;
     if (g.key_bits <= 32)
-        run_keyed<T, D, uint32_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm,
-                                  &stage);
+        run_keyed<T, D, uint32_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm);
     else
-        run_keyed<T, D, uint64_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm,
-                                  &stage);
+        run_keyed<T, D, uint64_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm);
     if (ctx.timing) {
         ctx.stats.ms_grid = tm.ms(0, 1);
         ctx.stats.ms_sort = tm.ms(1, 2);

@@ -10,17 +10,28 @@
 //   cluster of a core = 1 + smallest sorted position of a core in its
 //                       eps-connected component of cores
 //   border row: within eps of a core; noise: neither.
-// so the kernels compute exactly that:
-//   k_core      count within-eps candidates per row, stop at minPts
-//   k_union     union-find over core-core pairs (hook larger root under the
-//               smaller, so a root is the smallest position)
-//   k_flatten   root of every core; writes -(root+1) for core rows
-//   k_border    smallest adjacent root for non-core rows (0 = noise)
+// The kernels compute exactly that:
+//   k_core    count within-eps candidates per row, stop at minPts
+//   k_hook    every core points at the smallest adjacent core before it
+//             (a forest whose roots are local minima; parent[x] <= x)
+//   k_roots   root of every core (path-halving find), snapshot of the
+//             roots, and per occupied cell the smallest / largest root of
+//             its cores
+//   k_union   union-find over the core pairs not yet known to be connected:
+//             a neighbour row whose cells hold only cores of the chunk's
+//             single root is skipped, a candidate whose snapshot root equals
+//             the row's root is skipped, every other within-eps core pair
+//             is united (larger root hooked under the smaller)
+//   k_roots   final roots; core rows get -(root + 1)
+//   k_border  smallest adjacent root for non-core rows (0 = noise); a
+//             neighbourhood whose cells hold cores of one root only stops at
+//             the first hit
+// Each core pair is either tested or provably in one component already, so
+// the components, and with them every label, are exact.
 //
 // Work decomposition: one wave per 64-row chunk of an occupied cell; lanes
-// own the chunk's rows, the candidate rows stream through the wave with
-// wave-uniform addresses (the hardware broadcasts them), in the order of the
-// cell's neighbour ranges.
+// own the chunk's rows, the candidate rows stream through the wave in
+// aligned batches fetched with wave-uniform addresses.
 #include <climits>
 
 #include "common.hpp"
@@ -57,14 +68,16 @@ __device__ __forceinline__ Batch<U, B> load_batch(const U* __restrict__ base, in
     return *reinterpret_cast<const Batch<U, B>*>(base + a);
 }
 
-// Visit candidate rows [qb, qe).  body(q, qc, core_q) runs per candidate
-// (core_q is the candidate's core flag when `flags` is given, else 1);
-// stop() is checked after each batch and ends the visit when true.  Returns
-// true if stopped.
+// Visit candidate rows [qb, qe).  body(q, qc, core_q, aux_q) runs per
+// candidate; core_q is flags[q] (1 without flags), aux_q is aux[q] (0 without
+// aux).  stop() is checked after each batch and ends the visit when true.
+// Returns true if stopped.  soa, flags and aux are read up to the next
+// multiple of the batch size past qe (callers pad them to ld rows).
 template <typename T, int D, typename Body, typename Stop>
 __device__ __forceinline__ bool visit_range(const T* __restrict__ soa, int64_t ld,
-                                            const uint8_t* __restrict__ flags, int32_t qb,
-                                            int32_t qe, Body&& body, Stop&& stop) {
+                                            const uint8_t* __restrict__ flags,
+                                            const int32_t* __restrict__ aux, int32_t qb, int32_t qe,
+                                            Body&& body, Stop&& stop) {
     constexpr int B = kBatch<T>;
     for (int32_t a = qb & ~(B - 1); a < qe; a += B) {
         Batch<T, B> b[D];
@@ -72,25 +85,18 @@ __device__ __forceinline__ bool visit_range(const T* __restrict__ soa, int64_t l
         for (int j = 0; j < D; ++j) b[j] = load_batch<T, B>(soa + static_cast<int64_t>(j) * ld, a);
         Batch<uint8_t, B> f;
         if (flags) f = load_batch<uint8_t, B>(flags, a);
-        if (a >= qb && a + B <= qe) {
+        Batch<int32_t, B> x;
+        if (aux) x = load_batch<int32_t, B>(aux, a);
+        const bool full = a >= qb && a + B <= qe;
+        const int lo = qb > a ? qb - a : 0;
+        const int hi = qe - a < B ? qe - a : B;
 #pragma unroll
-            for (int t = 0; t < B; ++t) {
-                T qc[D];
+        for (int t = 0; t < B; ++t) {
+            if (!full && (t < lo || t >= hi)) continue;
+            T qc[D];
 #pragma unroll
-                for (int j = 0; j < D; ++j) qc[j] = b[j].v[t];
-                body(a + t, qc, flags ? f.v[t] : uint8_t(1));
-            }
-        } else {
-            const int lo = qb > a ? qb - a : 0;
-            const int hi = qe - a < B ? qe - a : B;
-#pragma unroll
-            for (int t = 0; t < B; ++t) {
-                if (t < lo || t >= hi) continue;
-                T qc[D];
-#pragma unroll
-                for (int j = 0; j < D; ++j) qc[j] = b[j].v[t];
-                body(a + t, qc, flags ? f.v[t] : uint8_t(1));
-            }
+            for (int j = 0; j < D; ++j) qc[j] = b[j].v[t];
+            body(a + t, qc, flags ? f.v[t] : uint8_t(1), aux ? x.v[t] : 0);
         }
         if (stop()) return true;
     }
@@ -98,17 +104,17 @@ __device__ __forceinline__ bool visit_range(const T* __restrict__ soa, int64_t l
 }
 
 struct Chunk {
-    int32_t u;      // occupied cell
-    int32_t first;  // first row of the chunk
-    int32_t end;    // end row of the cell
-    int32_t r0, r1; // its neighbour ranges
+    int32_t u;       // occupied cell
+    int32_t first;   // first row of the chunk
+    int32_t end;     // end row of the cell
+    int32_t r0, r1;  // its neighbour ranges (ascending)
 };
 
 __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int32_t nitems,
                                             const int32_t* __restrict__ ustart,
                                             const uint32_t* __restrict__ roff, Chunk& c) {
-    const int32_t wid =
-        uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock + static_cast<int32_t>(threadIdx.x >> 6));
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return false;
     const int2 it = items[wid];
     c.u = uniform(it.x);
@@ -119,114 +125,255 @@ __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int3
     return true;
 }
 
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Every occupied cell in [a, b) holds either no core or only cores of root R.
+__device__ __forceinline__ bool cells_single_root(int2 cs, const int32_t* __restrict__ cmin,
+                                                  const int32_t* __restrict__ cmax, int32_t R) {
+    for (int32_t c = cs.x; c < cs.y; ++c) {
+        const int32_t mn = cmin[c], mx = cmax[c];
+        if (mn <= mx && (mn != R || mx != R)) return false;
+    }
+    return true;
+}
+
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_core(const T* __restrict__ soa, int64_t ld,
-                                                   const int2* __restrict__ items, int32_t nitems,
-                                                   const int32_t* __restrict__ ustart,
-                                                   const uint32_t* __restrict__ roff,
-                                                   const int2* __restrict__ ranges, T eps2,
-                                                   uint32_t min_pts, uint8_t* __restrict__ core,
+__global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_t* __restrict__ core,
                                                    unsigned long long* __restrict__ ncore) {
     Chunk c;
-    if (!fetch_chunk(items, nitems, ustart, roff, c)) return;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     T pc[D];
-    load_row<T, D>(soa, ld, valid ? p : c.first, pc);
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
     uint32_t cnt = 0;
-    auto body = [&](int32_t, const T* qc, uint8_t) { cnt += within<D>(pc, qc, eps2) ? 1u : 0u; };
-    auto stop = [&] { return static_cast<bool>(__all(!valid || cnt >= min_pts)); };
+    // rows of a tight box with >= minPts rows are core without counting
+    if (a.btight && valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = a.min_pts;
+    auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
+        cnt += within<D>(pc, qc, a.eps2) ? 1u : 0u;
+    };
+    auto stop = [&] { return static_cast<bool>(__all(!valid || cnt >= a.min_pts)); };
     bool done = stop();
-    for (int32_t r = c.r0; r < c.r1 && !done; ++r) {
-        const int2 rg = ranges[r];
-        done = visit_range<T, D>(soa, ld, nullptr, uniform(rg.x), uniform(rg.y), body, stop);
+    // the cell's own row first: most neighbours are there
+    const int32_t center = uniform(a.rcenter[c.u]);
+    for (int32_t i = -1; i < c.r1 - c.r0 && !done; ++i) {
+        const int32_t r = i < 0 ? center : c.r0 + i;
+        if (i >= 0 && r == center) continue;
+        const int2 rg = a.ranges[r];
+        done = visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, uniform(rg.x), uniform(rg.y), body,
+                                 stop);
     }
-    const bool is_core = valid && cnt >= min_pts;
+    const bool is_core = valid && cnt >= a.min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
     const unsigned long long nc = __popcll(ballot(is_core));
     if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
 }
 
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_union(const T* __restrict__ soa, int64_t ld,
-                                                    const int2* __restrict__ items, int32_t nitems,
+__global__ __launch_bounds__(kThreads) void k_hook(const DbscanArgs<T> a,
+                                                   const uint8_t* __restrict__ core,
+                                                   int32_t* __restrict__ parent) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    const bool active = valid && core[p];
+    if (!__any(active)) return;
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+    // candidates arrive in ascending position, so the first hit is the
+    // smallest adjacent core before p
+    int32_t best = p;
+    bool found = !active;
+    auto body = [&](int32_t q, const T* qc, uint8_t cq, int32_t) {
+        if (!cq) return;
+        if (!found && q < p && within<D>(pc, qc, a.eps2)) {
+            best = q;
+            found = true;
+        }
+    };
+    auto stop = [&] { return static_cast<bool>(__all(found)); };
+    const int32_t pmax = uniform(min(c.first + kWave, c.end) - 1);
+    for (int32_t r = c.r0; r < c.r1; ++r) {
+        const int2 rg = a.ranges[r];
+        const int32_t qb = uniform(rg.x);
+        if (qb >= pmax) break;
+        if (visit_range<T, D>(a.soa, a.ld, core, nullptr, qb, uniform(min(rg.y, pmax)), body, stop))
+            break;
+    }
+    if (active) parent[p] = best;
+}
+
+// Roots of the chunk's cores (path-halving find), stored back into parent,
+// snapshot copy, and per-cell smallest / largest root.
+__global__ __launch_bounds__(kThreads) void k_roots(const int2* __restrict__ items, int32_t nitems,
                                                     const int32_t* __restrict__ ustart,
                                                     const uint32_t* __restrict__ roff,
-                                                    const int2* __restrict__ ranges, T eps2,
                                                     const uint8_t* __restrict__ core,
-                                                    int32_t* parent) {
+                                                    int32_t* parent, int32_t* __restrict__ rsnap,
+                                                    int32_t* __restrict__ cmin,
+                                                    int32_t* __restrict__ cmax) {
     Chunk c;
     if (!fetch_chunk(items, nitems, ustart, roff, c)) return;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     const bool active = valid && core[p];
     if (!__any(active)) return;
-    T pc[D];
-    load_row<T, D>(soa, ld, valid ? p : c.first, pc);
-    // every core pair is handled by its larger position: only q < p
-    const int32_t pmax = uniform(min(c.first + kWave, c.end) - 1);
-    auto body = [&](int32_t q, const T* qc, uint8_t cq) {
-        if (!cq) return;
-        const bool hit = active && q < p && within<D>(pc, qc, eps2);
-        if (hit) uf_unite(parent, p, q);
-    };
-    auto never = [] { return false; };
-    for (int32_t r = c.r0; r < c.r1; ++r) {
-        const int2 rg = ranges[r];
-        const int32_t qb = uniform(rg.x);
-        const int32_t qe = uniform(min(rg.y, pmax));
-        if (qb < qe) visit_range<T, D>(soa, ld, core, qb, qe, body, never);
+    int32_t r = INT_MAX;
+    if (active) {
+        r = uf_find(parent, p);
+        st_agent(&parent[p], r);
+        rsnap[p] = r;
+    }
+    const int32_t rmin = wave_min(active ? r : INT_MAX);
+    const int32_t rmax = wave_max(active ? r : -1);
+    if (lane_id() == 0) {
+        atomicMin(&cmin[c.u], rmin);
+        atomicMax(&cmax[c.u], rmax);
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_parent_init(int32_t* __restrict__ parent, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) parent[i] = static_cast<int32_t>(i);
-}
-
-__global__ __launch_bounds__(kThreads) void k_flatten(const uint8_t* __restrict__ core,
+// smallest sorted position of every slot-level component
+__global__ __launch_bounds__(kThreads) void k_compmin(const uint8_t* __restrict__ core,
                                                       int32_t* parent, int64_t n,
-                                                      const uint32_t* __restrict__ order,
-                                                      int64_t* __restrict__ labels) {
+                                                      const uint32_t* __restrict__ pos_of_slot,
+                                                      int32_t* __restrict__ compmin) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n || !core[i]) return;
     const int32_t r = uf_find(parent, static_cast<int32_t>(i));
-    parent[i] = r;
-    labels[order[i]] = -(static_cast<int64_t>(r) + 1);
+    const int32_t pos = pos_of_slot ? static_cast<int32_t>(pos_of_slot[i]) : static_cast<int32_t>(i);
+    atomicMin(&compmin[r], pos);
 }
 
-template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_border(const T* __restrict__ soa, int64_t ld,
-                                                     const int2* __restrict__ items, int32_t nitems,
-                                                     const int32_t* __restrict__ ustart,
-                                                     const uint32_t* __restrict__ roff,
-                                                     const int2* __restrict__ ranges, T eps2,
-                                                     const uint8_t* __restrict__ core,
-                                                     const int32_t* __restrict__ root,
-                                                     const uint32_t* __restrict__ order,
-                                                     int64_t* __restrict__ labels,
-                                                     unsigned long long* __restrict__ stats) {
+// cluster of every core slot -> label, slot array of clusters, per-cell
+// smallest / largest cluster
+__global__ __launch_bounds__(kThreads) void k_label_cores(
+    const int2* __restrict__ items, int32_t nitems, const int32_t* __restrict__ ustart,
+    const uint32_t* __restrict__ roff, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ box_of_slot, const int32_t* __restrict__ bcluster, int32_t* parent,
+    const int32_t* __restrict__ compmin, const uint32_t* __restrict__ inrow,
+    int64_t* __restrict__ labels, int32_t* __restrict__ rootslot, int32_t* __restrict__ cmin,
+    int32_t* __restrict__ cmax) {
     Chunk c;
     if (!fetch_chunk(items, nitems, ustart, roff, c)) return;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
-    const bool active = valid && !core[p];
+    const bool active = valid && core[p];
     if (!__any(active)) return;
+    int32_t r = INT_MAX;
+    if (active) {
+        r = bcluster ? bcluster[box_of_slot[p]] : compmin[uf_find(parent, p)];
+        labels[inrow[p]] = -(static_cast<int64_t>(r) + 1);
+        rootslot[p] = r;
+    }
+    const int32_t rmin = wave_min(active ? r : INT_MAX);
+    const int32_t rmax = wave_max(active ? r : -1);
+    if (lane_id() == 0) {
+        atomicMin(&cmin[c.u], rmin);
+        atomicMax(&cmax[c.u], rmax);
+    }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_union(const DbscanArgs<T> a,
+                                                    const uint8_t* __restrict__ core,
+                                                    const int32_t* __restrict__ rsnap,
+                                                    const int32_t* __restrict__ cmin,
+                                                    const int32_t* __restrict__ cmax,
+                                                    int32_t* parent) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    const bool active = valid && core[p];
+    if (!__any(active)) return;
+    const int32_t rp = active ? rsnap[p] : -1;
+    const int32_t rmin = wave_min(active ? rp : INT_MAX);
+    const int32_t rmax = wave_max(active ? rp : -1);
+    const int32_t R = uniform(rmin == rmax ? rmin : -1);  // the chunk's single root, if any
     T pc[D];
-    load_row<T, D>(soa, ld, valid ? p : c.first, pc);
-    int32_t best = INT_MAX, worst = -1;
-    auto body = [&](int32_t q, const T* qc, uint8_t cq) {
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+    // every core pair is handled by its larger position: only q < p
+    const int32_t pmax = uniform(min(c.first + kWave, c.end) - 1);
+    auto body = [&](int32_t q, const T* qc, uint8_t cq, int32_t rq) {
         if (!cq) return;
-        if (active && within<D>(pc, qc, eps2)) {
-            const int32_t rq = root[q];
-            best = min(best, rq);
-            worst = max(worst, rq);
-        }
+        const bool need = active && q < p && rq != rp;
+        if (!__any(need)) return;
+        if (need && within<D>(pc, qc, a.eps2)) uf_unite(parent, p, q);
     };
     auto never = [] { return false; };
     for (int32_t r = c.r0; r < c.r1; ++r) {
-        const int2 rg = ranges[r];
-        visit_range<T, D>(soa, ld, core, uniform(rg.x), uniform(rg.y), body, never);
+        const int2 rg = a.ranges[r];
+        const int32_t qb = uniform(rg.x);
+        if (qb >= pmax) break;
+        if (R >= 0 && cells_single_root(a.rcells[r], cmin, cmax, R)) continue;
+        visit_range<T, D>(a.soa, a.ld, core, rsnap, qb, uniform(min(rg.y, pmax)), body, never);
+    }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
+                                                     const uint8_t* __restrict__ core,
+                                                     const int32_t* __restrict__ root,
+                                                     const int32_t* __restrict__ cmin,
+                                                     const int32_t* __restrict__ cmax,
+                                                     const uint32_t* __restrict__ order,
+                                                     int64_t* __restrict__ labels,
+                                                     unsigned long long* __restrict__ stats) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    const bool active = valid && !core[p];
+    if (!__any(active)) return;
+    // do the candidate cells hold cores of a single root only?
+    int32_t R = -1;
+    bool single = true;
+    for (int32_t r = c.r0; r < c.r1 && single; ++r) {
+        const int2 cs = a.rcells[r];
+        for (int32_t cc = cs.x; cc < cs.y; ++cc) {
+            const int32_t mn = cmin[cc], mx = cmax[cc];
+            if (mn > mx) continue;
+            if (mn != mx || (R >= 0 && R != mn)) {
+                single = false;
+                break;
+            }
+            R = mn;
+        }
+    }
+    R = uniform(R);
+    int32_t best = INT_MAX, worst = -1;
+    if (!(single && R < 0)) {  // otherwise no core is near: all noise
+        T pc[D];
+        load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+        auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t rq) {
+            if (!cq) return;
+            if (active && within<D>(pc, qc, a.eps2)) {
+                best = min(best, rq);
+                worst = max(worst, rq);
+            }
+        };
+        auto stop = [&] {
+            return single && static_cast<bool>(__all(!active || best != INT_MAX));
+        };
+        const int32_t center = uniform(a.rcenter[c.u]);
+        for (int32_t i = -1; i < c.r1 - c.r0; ++i) {
+            const int32_t r = i < 0 ? center : c.r0 + i;
+            if (i >= 0 && r == center) continue;
+            const int2 rg = a.ranges[r];
+            if (visit_range<T, D>(a.soa, a.ld, core, root, uniform(rg.x), uniform(rg.y), body,
+                                  stop))
+                break;
+        }
     }
     if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) + 1;
     const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
@@ -234,6 +381,20 @@ __global__ __launch_bounds__(kThreads) void k_border(const T* __restrict__ soa, 
     if (lane_id() == 0) {
         if (noise) atomicAdd(&stats[0], noise);
         if (amb) atomicAdd(&stats[1], amb);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_iota(int32_t* __restrict__ v, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) v[i] = static_cast<int32_t>(i);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill2(int32_t* __restrict__ a, int32_t va,
+                                                    int32_t* __restrict__ b, int32_t vb, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) {
+        a[i] = va;
+        b[i] = vb;
     }
 }
 
@@ -245,46 +406,57 @@ template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s) {
     if (a.nitems <= 0) return;
     hipLaunchKernelGGL((k_core<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)), dim3(kThreads),
-                       0, s, a.soa, a.ld, a.items, a.nitems, a.ustart, a.roff, a.ranges, a.eps2,
-                       a.min_pts, core, ncore);
+                       0, s, a, core, ncore);
     HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D>
-void launch_union(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent, hipStream_t s) {
+void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent,
+                         int32_t* rsnap, int32_t* cmin, int32_t* cmax, int32_t* compmin,
+                         hipStream_t s) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_parent_init, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, parent,
-                       a.n);
-    if (a.nitems > 0)
-        hipLaunchKernelGGL((k_union<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, a.soa, a.ld, a.items, a.nitems, a.ustart, a.roff,
-                           a.ranges, a.eps2, core, parent);
+    const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, parent, a.n);
+    hipLaunchKernelGGL((k_hook<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core, parent);
+    hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
+                       cmax, -1, static_cast<int64_t>(a.m));
+    hipLaunchKernelGGL(k_roots, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems, a.ustart, a.roff,
+                       core, parent, rsnap, cmin, cmax);
+    hipLaunchKernelGGL((k_union<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
+                       static_cast<const int32_t*>(rsnap), static_cast<const int32_t*>(cmin),
+                       static_cast<const int32_t*>(cmax), parent);
+    hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, compmin,
+                       INT_MAX, rsnap, 0, a.n);
+    hipLaunchKernelGGL(k_compmin, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, core,
+                       parent, a.n, a.pos_of_slot, compmin);
     HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D>
-void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent,
-                   const uint32_t* order, int64_t* labels, unsigned long long* stats,
-                   hipStream_t s) {
+void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                   int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
+                   int32_t* cmax, int64_t* labels, unsigned long long* stats, hipStream_t s) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_flatten, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, core,
-                       parent, a.n, order, labels);
-    if (a.nitems > 0)
-        hipLaunchKernelGGL((k_border<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, a.soa, a.ld, a.items, a.nitems, a.ustart, a.roff,
-                           a.ranges, a.eps2, core, static_cast<const int32_t*>(parent), order,
-                           labels, stats);
+    const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
+    hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
+                       cmax, -1, static_cast<int64_t>(a.m));
+    hipLaunchKernelGGL(k_label_cores, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems, a.ustart,
+                       a.roff, core, a.box_of_slot, bcluster, parent, compmin, a.inrow, labels,
+                       rootslot, cmin, cmax);
+    hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
+                       static_cast<const int32_t*>(rootslot), static_cast<const int32_t*>(cmin),
+                       static_cast<const int32_t*>(cmax), a.inrow, labels, stats);
     HPD_HIP(hipGetLastError());
 }
 
-#define HPD_INST_DBSCAN(T, D)                                                                  \
-    template void launch_core<T, D>(const DbscanArgs<T>&, uint8_t*, unsigned long long*,       \
-                                    hipStream_t);                                              \
-    template void launch_union<T, D>(const DbscanArgs<T>&, const uint8_t*, int32_t*,           \
-                                     hipStream_t);                                             \
-    template void launch_labels<T, D>(const DbscanArgs<T>&, const uint8_t*, int32_t*,          \
-                                      const uint32_t*, int64_t*, unsigned long long*,          \
-                                      hipStream_t);
+#define HPD_INST_DBSCAN(T, D)                                                                     \
+    template void launch_core<T, D>(const DbscanArgs<T>&, uint8_t*, unsigned long long*,          \
+                                    hipStream_t);                                                 \
+    template void launch_union_points<T, D>(const DbscanArgs<T>&, const uint8_t*, int32_t*,       \
+                                            int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t); \
+    template void launch_labels<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \
+                                      int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*,     \
+                                      int64_t*, unsigned long long*, hipStream_t);
 #define HPD_INST_DBSCAN_ALL_D(T) \
     HPD_INST_DBSCAN(T, 1)        \
     HPD_INST_DBSCAN(T, 2)        \

@@ -18,23 +18,39 @@ struct DbscanArgs {
     int32_t nitems = 0;
     const int32_t* ustart = nullptr; // first row of each occupied cell, [m+1]
     const uint32_t* roff = nullptr;  // CSR offsets into ranges, [m+1]
-    const int2* ranges = nullptr;    // [first, end) rows of neighbour rows
+    const int2* ranges = nullptr;    // [first, end) rows of neighbour rows, ascending
+    const int2* rcells = nullptr;    // [first, end) occupied cells of the same rows
+    const int32_t* rcenter = nullptr; // index into ranges of each cell's own row
+    int32_t m = 0;                   // occupied cells
     T eps2{};                        // EPS2 converted to T
     uint32_t min_pts = 0;
+    // slot order: rows sorted by cell, then (when boxes are used) by sub-box
+    const uint32_t* pos_of_slot = nullptr;  // sorted position per slot (null: identity)
+    const uint32_t* inrow = nullptr;        // input row per slot
+    const int32_t* box_of_slot = nullptr;   // box per slot (null: no boxes)
+    const uint8_t* btight = nullptr;        // box flags (bit 1: all rows core)
 };
 
 template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s);
 
+// Components of the core-core graph on slots (used when boxes are not):
+// hook forest, root snapshot, union of the pairs not yet connected, then
+// compmin[root] = smallest sorted position of each component.
+// parent, rsnap: ld entries; cmin, cmax: m; compmin: n.
 template <typename T, int D>
-void launch_union(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent, hipStream_t s);
+void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent,
+                         int32_t* rsnap, int32_t* cmin, int32_t* cmax, int32_t* compmin,
+                         hipStream_t s);
 
-// roots of the cores, then border / noise labels; labels are written in
-// input order through `order` (sorted position -> input row).
-// stats[0] += noise rows, stats[1] += ambiguous border rows
+// Labels: core slots get -(cluster + 1), where cluster comes from bcluster
+// (box mode, bcluster != null) or compmin[find(parent, slot)]; then border
+// and noise labels.  Labels go to input order through a.inrow.
+// rootslot: ld entries.  stats[0] += noise rows, stats[1] += ambiguous
+// border rows.
 template <typename T, int D>
-void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent,
-                   const uint32_t* order, int64_t* labels, unsigned long long* stats,
-                   hipStream_t s);
+void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                   int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
+                   int32_t* cmax, int64_t* labels, unsigned long long* stats, hipStream_t s);
 
 }  // namespace hpd

@@ -93,17 +93,45 @@ __global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restri
 }
 
 template <typename T, int D>
-__global__ void k_minmax_final(const T* __restrict__ part, int nblocks, T* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(kMMThreads) void k_minmax_final(const T* __restrict__ part, int nblocks,
+                                                             T* __restrict__ out) {
+    __shared__ T smn[D][kMMThreads];
+    __shared__ T smx[D][kMMThreads];
+    T mn[D], mx[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        T mn = min_init<T>(), mx = max_init<T>();
-        for (int b = 0; b < nblocks; ++b) {
-            mn = ref_min(mn, part[static_cast<int64_t>(b) * 2 * D + j]);
-            mx = ref_max(mx, part[static_cast<int64_t>(b) * 2 * D + D + j]);
+        mn[j] = min_init<T>();
+        mx[j] = max_init<T>();
+    }
+    for (int b = threadIdx.x; b < nblocks; b += kMMThreads) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            mn[j] = ref_min(mn[j], part[static_cast<int64_t>(b) * 2 * D + j]);
+            mx[j] = ref_max(mx[j], part[static_cast<int64_t>(b) * 2 * D + D + j]);
         }
-        out[j] = mn;
-        out[D + j] = mx;
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        smn[j][threadIdx.x] = mn[j];
+        smx[j][threadIdx.x] = mx[j];
+    }
+    __syncthreads();
+    for (int o = kMMThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                smn[j][threadIdx.x] = ref_min(smn[j][threadIdx.x], smn[j][threadIdx.x + o]);
+                smx[j][threadIdx.x] = ref_max(smx[j][threadIdx.x], smx[j][threadIdx.x + o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            out[j] = smn[j][0];
+            out[D + j] = smx[j][0];
+        }
     }
 }
 
@@ -205,7 +233,7 @@ void launch_minmax(const T* pts, int64_t n, T* part, T* out, int* nonfinite, hip
     const int nb = minmax_partial_blocks(n);
     hipLaunchKernelGGL((k_minmax_partial<T, D>), dim3(nb), dim3(kMMThreads), 0, s, pts, n, part,
                        nonfinite);
-    hipLaunchKernelGGL((k_minmax_final<T, D>), dim3(1), dim3(64), 0, s, part, nb, out);
+    hipLaunchKernelGGL((k_minmax_final<T, D>), dim3(1), dim3(kMMThreads), 0, s, part, nb, out);
     HPD_HIP(hipGetLastError());
 }
 

@@ -13,7 +13,8 @@
 // the up-to-three cells (i0-1, i0, i0+1) of one "row" occupy one contiguous
 // range of sorted rows.  A cell therefore has at most 3^(D-1) ranges, found
 // by binary search of the row's first and last cell id in the occupied-cell
-// table.  The row holding the cell itself comes first.
+// table.  A cell's ranges are stored in ascending row order; the index of
+// the row holding the cell itself is kept separately.
 #include "common.hpp"
 #include "index.hpp"
 #include "internal.hpp"
@@ -71,24 +72,25 @@ __device__ __forceinline__ void decode_cell(uint64_t key, const CellParams& cp, 
     }
 }
 
-// Visit the neighbour rows of a cell.  fn(row_lo_key, row_hi_key) gets the
-// inclusive cell-id bounds of each row that lies inside the grid.  Row order:
-// the cell's own row first, then all others in odometer order.
+// Visit the neighbour rows of a cell in ascending cell-id order.
+// fn(row_lo_key, row_hi_key, is_center) gets the inclusive cell-id bounds of
+// each row that lies inside the grid.  Odometer order over the offsets of
+// swapped dimensions 1..D-1, most significant dimension slowest, is ascending
+// because the offsets of the lower dimensions never reach one stride of the
+// next.
 template <typename F>
 __device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp, F&& fn) {
     uint64_t idx[kMaxDims];
     decode_cell(key, cp, idx);
     const uint64_t lo0 = idx[0] > 0 ? 1 : 0;
     const uint64_t hi0 = idx[0] + 1 < cp.ncell[0] ? 1 : 0;
-    // the cell's own row
-    fn(key - lo0, key + hi0);
     int nrows = 1;
     for (int k = 1; k < cp.d; ++k) nrows *= 3;
+    const int center_row = (nrows - 1) / 2;  // every digit 1: offset 0 in all dims
     for (int r = 0; r < nrows; ++r) {
         // digit k-1 of r in base 3 is the offset (0 -> -1, 1 -> 0, 2 -> +1)
         int rem = r;
         bool inside = true;
-        bool center = true;
         uint64_t base = key;
         for (int k = 1; k < cp.d; ++k) {
             const int dgt = rem % 3;
@@ -96,15 +98,13 @@ __device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp,
             if (dgt == 0) {
                 if (idx[k] == 0) inside = false;
                 base -= cp.stride[k];
-                center = false;
             } else if (dgt == 2) {
                 if (idx[k] + 1 >= cp.ncell[k]) inside = false;
                 base += cp.stride[k];
-                center = false;
             }
         }
-        if (!inside || center) continue;
-        fn(base - lo0, base + hi0);
+        if (!inside) continue;
+        fn(base - lo0, base + hi0, r == center_row);
     }
 }
 
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
     uint64_t count_u = 0;
     if (u < m) {
         uint32_t cnt = 0;
-        for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi) {
+        for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi, bool) {
             const int32_t a = lower_bound_cells(ucell, m, lo);
             const int32_t b = lower_bound_cells(ucell, m, hi + 1);
             if (b > a) {
@@ -142,15 +142,23 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
                                                             const int32_t* __restrict__ ustart,
                                                             int32_t m, CellParams cp,
                                                             const uint32_t* __restrict__ roff,
-                                                            int2* __restrict__ ranges) {
+                                                            int2* __restrict__ ranges,
+                                                            int2* __restrict__ rcells,
+                                                            int32_t* __restrict__ rcenter) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     uint32_t w = roff[u];
-    for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi) {
+    int32_t center = static_cast<int32_t>(w);
+    for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi, bool is_center) {
         const int32_t a = lower_bound_cells(ucell, m, lo);
         const int32_t b = lower_bound_cells(ucell, m, hi + 1);
-        if (b > a) ranges[w++] = make_int2(ustart[a], ustart[b]);
+        if (b > a) {
+            if (is_center) center = static_cast<int32_t>(w);
+            rcells[w] = make_int2(a, b);
+            ranges[w++] = make_int2(ustart[a], ustart[b]);
+        }
     });
+    rcenter[u] = center;
 }
 
 __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
@@ -194,9 +202,10 @@ void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, con
 
 template <typename K>
 void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
-                          const uint32_t* roff, int2* ranges, hipStream_t s) {
+                          const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
+                          hipStream_t s) {
     hipLaunchKernelGGL(k_neighbor_rows_fill<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, ucell,
-                       ustart, m, cp, roff, ranges);
+                       ustart, m, cp, roff, ranges, rcells, rcenter);
     HPD_HIP(hipGetLastError());
 }
 
@@ -219,7 +228,7 @@ void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, i
     template void launch_neighbor_count<K>(const K*, const int32_t*, int32_t, const CellParams&, \
                                            uint32_t*, unsigned long long*, hipStream_t);         \
     template void launch_neighbor_fill<K>(const K*, const int32_t*, int32_t, const CellParams&,  \
-                                          const uint32_t*, int2*, hipStream_t);
+                                          const uint32_t*, int2*, int2*, int32_t*, hipStream_t);
 HPD_INST_INDEX(uint32_t)
 HPD_INST_INDEX(uint64_t)
 

@@ -21,10 +21,13 @@ template <typename K>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
                            uint32_t* nranges, unsigned long long* sumcand, hipStream_t s);
 
-// ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row
+// ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row,
+// ascending; rcells[..] = [first, end) occupied-cell index of the same row;
+// rcenter[u] = index (into ranges) of the row holding cell u itself
 template <typename K>
 void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
-                          const uint32_t* roff, int2* ranges, hipStream_t s);
+                          const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
+                          hipStream_t s);
 
 void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s);
 void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,

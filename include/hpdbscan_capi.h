@@ -86,6 +86,8 @@ typedef struct hpd_run_stats {
     double ms_total;             /* first to last event */
     int32_t core_launches;       /* launches of the core-count kernel */
     double ms_core_kernel;       /* summed duration of those launches */
+    uint64_t boxes;              /* sub-boxes of the occupied cells */
+    int32_t box_union;           /* 1 if the union ran on sub-boxes */
 } hpd_run_stats;
 
 /* Thread-local message describing the last failure ("" if none). */

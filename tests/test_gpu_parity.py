@@ -156,3 +156,35 @@ def test_nonfinite_input_rejected(ctx):
     x[5, 1] = np.inf
     with pytest.raises(ValueError):
         ctx.cluster(x, 0.3, 3)
+
+
+def test_loose_boxes_fall_back_to_point_union(ctx):
+    """x near 6e6 with one row at 0 and eps 0.3: (x - min) * (1/eps) exceeds
+    2^24, its float spacing is 2, cells are 2 eps wide and some sub-boxes are
+    not tight; the slot-level union must take over and stay exact."""
+    rng = np.random.default_rng(3)
+    base = np.float32(6.0e6)
+    x = np.stack([base + rng.integers(0, 4000, 6000).astype(np.float32) * np.float32(0.5),
+                  (rng.integers(0, 3, 6000) * 0.1).astype(np.float32)], axis=1).astype(np.float32)
+    x = np.concatenate([x, np.zeros((1, 2), np.float32)])
+    say("loose boxes n=6001")
+    got = ctx.cluster(x, 0.3, 3)
+    assert ctx.stats()["box_union"] == 0
+    assert_parity(x, 0.3, 3, got, oracle.cluster(x, 0.3, 3))
+
+
+def test_box_union_used_and_exact(ctx):
+    x = blobs(40000, 3, 20, 0.15, 8.0, 21, np.float32)
+    say("box union n=40000")
+    got = ctx.cluster(x, 0.3, 8)
+    st = ctx.stats()
+    assert st["box_union"] == 1 and st["boxes"] > 0
+    assert_parity(x, 0.3, 8, got, oracle.cluster(x, 0.3, 8))
+
+
+def test_skewed_2d_sample(ctx):
+    from hpdbscan_amd import datagen
+    x = datagen.skewed_2d(n=200_000, seed=3)
+    say("skewed 2-D sample n=200000 eps=0.01 minPts=32")
+    got = ctx.cluster(x, 0.01, 32)
+    assert_parity(x, 0.01, 32, got, oracle.cluster(x, 0.01, 32))
