@@ -18,6 +18,13 @@ The JSON line also carries
                 x0 < X, which keeps the density (and so the per-row work) of
                 the full workload
   parity        GPU labels vs the reference's labels on that sample
+  batched       the batched workload (4096 clouds, 1k-200k rows each) in one
+                hpd_cluster_batch_device call vs a loop of one call per
+                cloud: jobs/s of both, per-call p50/p99 of the loop, and
+                equality of the labels
+  serving       a Poisson stream (200 requests/s, 10k-5M rows, d in 2..4)
+                through the continuous-batching server: offered vs achieved
+                load, p50/p99 latency
 Only one GPU is used; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
@@ -86,6 +93,107 @@ def run_reference(sample: np.ndarray, eps: float, mp: int, threads: int, reps: i
     return res, labels, None
 
 
+def batched_leg(a) -> dict:
+    """All clouds of the batched workload in one call vs one call per cloud."""
+    t0 = time.time()
+    rows, off, eps, mps = datagen.batch_clouds(n_jobs=a.batch_jobs)
+    nj = len(off) - 1
+    print(f"[bench] batched: {nj} clouds, {len(rows)} rows (generated in {time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    ctx = _lib.Context(0)
+    xd = _lib.DeviceBuffer.from_array(ctx, rows)
+    ld = _lib.DeviceBuffer(ctx, 8 * len(rows))
+    # warm up both paths (allocations)
+    ctx.cluster_batch_device(xd.ptr, off, 3, np.float32, eps, mps, ld.ptr)
+    ctx.synchronize()
+    reps = []
+    for _ in range(a.batch_reps):
+        t = time.perf_counter()
+        ctx.cluster_batch_device(xd.ptr, off, 3, np.float32, eps, mps, ld.ptr)
+        ctx.synchronize()
+        reps.append(time.perf_counter() - t)
+    batched_labels = ld.to_array(np.int64, len(rows))
+    t_batch = float(np.median(reps))
+    # the plain loop: one hpd_cluster_device per cloud, same device buffers
+    lat = []
+    for j in range(nj):
+        n = int(off[j + 1] - off[j])
+        t = time.perf_counter()
+        ctx.cluster_device(xd.ptr + 12 * int(off[j]), n, 3, np.float32, float(eps[j]), int(mps[j]),
+                           ld.ptr + 8 * int(off[j]))
+        lat.append(time.perf_counter() - t)
+    ctx.synchronize()
+    plain_labels = ld.to_array(np.int64, len(rows))
+    t_plain = float(sum(lat))
+    equal = bool(np.array_equal(batched_labels, plain_labels))
+    xd.close()
+    ld.close()
+    ctx.close()
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "workload": "batch_clouds", "jobs": nj, "rows": int(len(rows)),
+        "batched_jobs_per_s": round(nj / t_batch, 1), "batched_s": round(t_batch, 4),
+        "batched_mpoints_per_s": round(len(rows) / t_batch / 1e6, 1),
+        "plain_jobs_per_s": round(nj / t_plain, 1), "plain_s": round(t_plain, 3),
+        "plain_p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "plain_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "batched_latency_ms": round(t_batch * 1e3, 2),
+        "speedup_jobs_per_s": round(t_plain / t_batch, 2),
+        "labels_equal": equal,
+        "note": "device-resident rows; a batched job's latency is the whole batch's",
+    }
+
+
+def serving_leg(a) -> dict:
+    """Poisson arrivals into the continuous-batching server."""
+    import threading
+    t0 = time.time()
+    stream = datagen.serving_stream(n_requests=a.serve_requests, rate=a.serve_rate)
+    offered_rows = sum(len(r[1]) for r in stream)
+    span = stream[-1][0] if stream else 0.0
+    print(f"[bench] serving: {len(stream)} requests over {span:.2f}s, {offered_rows} rows "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    srv = _lib.Server(lanes=a.serve_lanes)
+    # warm-up request per dimension (contexts, allocations)
+    for d in (2, 3, 4):
+        w = next(r for r in stream if r[1].shape[1] == d)
+        srv.wait(srv.submit(w[1], w[2], w[3]))
+    srv.close()
+    srv = _lib.Server(lanes=a.serve_lanes)
+    tickets = [None] * len(stream)
+    start = time.perf_counter()
+
+    def producer():
+        for i, (arr, x, e, m) in enumerate(stream):
+            delay = start + arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(x, e, m)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    infos = []
+    for t in tickets:
+        _, info = srv.wait(t)
+        infos.append(info)
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    lat = np.array([i["latency_ms"] for i in infos])
+    return {
+        "workload": "serving_stream", "requests": len(stream), "lanes": a.serve_lanes,
+        "offered_jobs_per_s": round(len(stream) / span, 1) if span > 0 else None,
+        "offered_mpoints_per_s": round(offered_rows / span / 1e6, 1) if span > 0 else None,
+        "achieved_jobs_per_s": round(len(stream) / wall, 1),
+        "achieved_mpoints_per_s": round(offered_rows / wall / 1e6, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "max_ms": round(float(lat.max()), 2),
+        "batches": int(st["batches"]), "mean_batch_jobs": round(st["mean_batch_jobs"], 2),
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +206,13 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=500_000,
                     help="rows in the CPU-baseline slab")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=4096)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-lanes", type=int, default=3)
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -119,27 +234,24 @@ def main() -> None:
     def step():
         ctx.cluster_device(xd.ptr, n, d, np.float32, eps, mp, ld.ptr)
 
-    ctx.set_timing(False)
+    # phase times come from HIP events the library records on its own stream
+    # (the one every kernel is launched on) during the timed steps themselves
+    ctx.set_timing(True)
     for _ in range(a.warmup):
         step()
     ctx.synchronize()
+    phases = []
     t_start = time.perf_counter()
     for _ in range(a.steps):
         step()
+        phases.append(ctx.stats())
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
+    ctx.set_timing(False)
     ms_per_step = 1e3 * elapsed / a.steps
     value = n * a.steps / elapsed / 1e6
     print(f"[bench] {a.steps} steps in {elapsed:.3f}s: {ms_per_step:.2f} ms/step, "
           f"{value:.1f} Mpoints/s", file=sys.stderr, flush=True)
-
-    # phase times (HIP events on the library's stream), separate timed steps
-    ctx.set_timing(True)
-    phases = []
-    for _ in range(max(3, min(a.steps, 5))):
-        step()
-        phases.append(ctx.stats())
-    ctx.set_timing(False)
     st = phases[-1]
     keys = ["ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union", "ms_border", "ms_total"]
     med = {k: float(np.median([p[k] for p in phases])) for k in keys}
@@ -207,11 +319,15 @@ def main() -> None:
             }
             result["speedup_vs_cpu"] = round(value / cpu_v, 1) if cpu_v > 0 else None
 
-    print(json.dumps(result), flush=True)
     xd.close()
     ld.close()
     ctx.close()
     del labels
+    if not a.no_batched:
+        result["batched"] = batched_leg(a)
+    if not a.no_serving:
+        result["serving"] = serving_leg(a)
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -13,7 +13,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import (BORDER_MIN_CLUSTER, BORDER_ONE_THREAD, Context, HPDError,  # noqa: F401
-                   default_context, load)
+                   Server, concat_jobs, default_context, load)
 from .hpdbscan import HPDBSCAN  # noqa: F401
 
 __version__ = "0.1.0"

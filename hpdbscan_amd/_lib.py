@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhpdbscan_amd.so")
 
-HPD_OK, HPD_EINVAL, HPD_ENOMEM, HPD_EDEVICE, HPD_EUNSUPPORTED = 0, 1, 2, 3, 4
+HPD_OK, HPD_EINVAL, HPD_ENOMEM, HPD_EDEVICE, HPD_EUNSUPPORTED, HPD_ETIMEOUT = 0, 1, 2, 3, 4, 5
 
 BORDER_MIN_CLUSTER = 0
 BORDER_ONE_THREAD = 1
@@ -63,6 +63,41 @@ class RunStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    """Mirror of hpd_server_config."""
+
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("lanes", ctypes.c_int),
+        ("hbm_budget_bytes", ctypes.c_size_t),
+        ("bytes_per_row_estimate", ctypes.c_size_t),
+        ("max_batch_jobs", ctypes.c_int32),
+        ("max_batch_rows", ctypes.c_int64),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int64), ("batch_jobs", ctypes.c_int32),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+        ("mean_batch_jobs", ctypes.c_double), ("rows", ctypes.c_int64),
+        ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+        ("wall_s", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
+        ("mrows_per_s", ctypes.c_double), ("queued", ctypes.c_uint64),
+        ("charged_bytes", ctypes.c_size_t),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # every function of include/hpdbscan_capi.h: name -> (restype, argtypes)
 _vp = ctypes.c_void_p
 SIGNATURES = {
@@ -78,12 +113,26 @@ SIGNATURES = {
                                    ctypes.c_float, ctypes.c_uint64, _vp]),
     "hpd_cluster_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.c_float, ctypes.c_uint64, _vp, _vp]),
+    "hpd_cluster_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                         _vp, _vp, _vp]),
+    "hpd_cluster_batch_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int32,
+                                                ctypes.c_int32, _vp, _vp, _vp, _vp]),
     "hpd_device_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "hpd_device_free": (ctypes.c_int, [_vp, _vp]),
     "hpd_memcpy_to_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "hpd_memcpy_to_host": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "hpd_context_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "hpd_context_synchronize": (ctypes.c_int, [_vp]),
+    "hpd_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+    "hpd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    "hpd_server_submit": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
+                                         ctypes.c_float, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "hpd_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, ctypes.c_int64, ctypes.c_double,
+                                       ctypes.POINTER(RequestInfo)]),
+    "hpd_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "hpd_server_destroy": (ctypes.c_int, [_vp]),
+    "hpd_server_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None
@@ -194,6 +243,53 @@ class Context:
                                         _vp(stream) if stream else None))
 
 
+def _batch_args(offsets, eps, min_points):
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nj = len(off) - 1
+    e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float32), (nj,)))
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(min_points, dtype=np.uint64), (nj,)))
+    return off, nj, e, m
+
+
+def concat_jobs(clouds):
+    """(rows, offsets) of a list of 2-D arrays of one dtype and width."""
+    clouds = [np.ascontiguousarray(c) for c in clouds]
+    off = np.zeros(len(clouds) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    if not clouds:
+        return np.zeros((0, 1), np.float32), off
+    return np.ascontiguousarray(np.concatenate(clouds)), off
+
+
+def _cluster_batch(self, data: np.ndarray, offsets, eps, min_points) -> np.ndarray:
+    """Labels of several clouds at once: rows [offsets[j], offsets[j+1]) form
+    job j with eps[j] / min_points[j] (scalars apply to every job)."""
+    a = np.ascontiguousarray(data)
+    if a.ndim != 2:
+        raise TypeError("data must be 2-D")
+    if a.dtype not in DTYPE_CODES:
+        raise TypeError(f"unsupported dtype {a.dtype}")
+    off, nj, e, m = _batch_args(offsets, eps, min_points)
+    if nj > 0 and off[-1] != a.shape[0]:
+        raise ValueError("offsets[-1] must equal the number of rows")
+    out = np.empty(a.shape[0], dtype=np.int64)
+    check(load().hpd_cluster_batch(self._h, DTYPE_CODES[a.dtype], a.ctypes.data, off.ctypes.data, nj,
+                                   a.shape[1], e.ctypes.data, m.ctypes.data, out.ctypes.data))
+    return out
+
+
+def _cluster_batch_device(self, data_ptr: int, offsets, d: int, dtype, eps, min_points,
+                          labels_ptr: int, stream: int = 0) -> None:
+    off, nj, e, m = _batch_args(offsets, eps, min_points)
+    check(load().hpd_cluster_batch_device(self._h, DTYPE_CODES[np.dtype(dtype)], _vp(data_ptr),
+                                          off.ctypes.data, nj, d, e.ctypes.data, m.ctypes.data,
+                                          _vp(labels_ptr), _vp(stream) if stream else None))
+
+
+Context.cluster_batch = _cluster_batch
+Context.cluster_batch_device = _cluster_batch_device
+
+
 class DeviceBuffer:
     """Device memory owned through a context (freed on close / GC)."""
 
@@ -243,3 +339,84 @@ def default_context(device: int | None = None) -> Context:
             _default.setdefault(device, ctx)
             ctx = _default[device]
     return ctx
+
+
+def _check_server(code: int) -> None:
+    if code == HPD_OK:
+        return
+    msg = (load().hpd_server_last_error() or b"").decode(errors="replace")
+    if code == HPD_EINVAL:
+        raise ValueError(msg)
+    if code == HPD_ENOMEM:
+        raise MemoryError(msg)
+    if code == HPD_ETIMEOUT:
+        raise TimeoutError(msg)
+    raise HPDError(code, msg)
+
+
+class Server:
+    """Continuous-batching clustering server (hpd_server_*)."""
+
+    def __init__(self, device: int = 0, lanes: int = 3, hbm_budget_bytes: int | None = None,
+                 bytes_per_row_estimate: int | None = None, max_batch_jobs: int | None = None,
+                 max_batch_rows: int | None = None):
+        lib = load()
+        cfg = ServerConfig()
+        lib.hpd_server_default_config(ctypes.byref(cfg))
+        cfg.device = device
+        cfg.lanes = lanes
+        if hbm_budget_bytes is not None:
+            cfg.hbm_budget_bytes = hbm_budget_bytes
+        if bytes_per_row_estimate is not None:
+            cfg.bytes_per_row_estimate = bytes_per_row_estimate
+        if max_batch_jobs is not None:
+            cfg.max_batch_jobs = max_batch_jobs
+        if max_batch_rows is not None:
+            cfg.max_batch_rows = max_batch_rows
+        h = _vp()
+        _check_server(lib.hpd_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def submit(self, data: np.ndarray, eps: float, min_points: int) -> int:
+        a = np.ascontiguousarray(data)
+        if a.ndim != 2 or a.dtype not in DTYPE_CODES:
+            raise TypeError("data must be a 2-D array of a supported dtype")
+        t = ctypes.c_uint64()
+        _check_server(load().hpd_server_submit(self._h, DTYPE_CODES[a.dtype], a.ctypes.data,
+                                               a.shape[0], a.shape[1], ctypes.c_float(eps),
+                                               int(min_points), ctypes.byref(t)))
+        self._n[t.value] = a.shape[0]
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """(labels, info dict) of a ticket."""
+        n = self._n[ticket]
+        out = np.empty(n, dtype=np.int64)
+        info = RequestInfo()
+        _check_server(load().hpd_server_wait(self._h, ticket, out.ctypes.data, n, timeout_ms,
+                                             ctypes.byref(info)))
+        del self._n[ticket]
+        return out, info.as_dict()
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check_server(load().hpd_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().hpd_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -3,7 +3,8 @@
     python -m hpdbscan_amd.build [--jobs N] [--verbose]
 
 Objects go to hpdbscan_amd/_build/, the library next to this file so that it
-travels with the source tree.  Compiler flags that matter for the results:
+travels with the source tree, and so does `copy_probe` (a plain copy kernel
+of known size that calibrates the HBM traffic counters).  Compiler flags that matter for the results:
 
   -ffp-contract=off                no fused multiply-add: the squared distance
                                    is evaluated with separate roundings, as
@@ -25,10 +26,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhpdbscan_amd.so")
+COPY_PROBE = os.path.join(HERE, "copy_probe")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["capi.hip", "scan.hip", "radix_sort.hip", "grid.hip", "index.hip", "dbscan.hip",
-           "boxes.hip"]
+           "boxes.hip", "server.cpp"]
 
 ARCH = os.environ.get("HPD_OFFLOAD_ARCH", "gfx950")
 
@@ -57,7 +59,10 @@ def compile_flags() -> list[str]:
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *compile_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    flags = compile_flags() if src.endswith(".hip") else [
+        "-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-pthread"]
+    cmd = [hipcc(), *flags, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,12 +82,26 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_copy_probe(verbose)
     return LIB
+
+
+def build_copy_probe(verbose: bool = False) -> str:
+    """The copy kernel of known size that calibrates the HBM traffic counters."""
+    tmp = COPY_PROBE + ".tmp"
+    cmd = [hipcc(), *compile_flags(), "-x", "hip", os.path.join(CSRC, "copy_probe.cpp"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on copy_probe.cpp:\n{r.stderr}")
+    os.replace(tmp, COPY_PROBE)
+    return COPY_PROBE
 
 
 def main() -> None:

@@ -60,23 +60,26 @@ __global__ __launch_bounds__(kThreads) void k_box_keys(const T* __restrict__ pts
                                                        const uint32_t* __restrict__ inrow_of_pos,
                                                        const uint32_t* __restrict__ flags,
                                                        const uint32_t* __restrict__ scan,
-                                                       const T* __restrict__ minv, float rcp_f,
-                                                       double rcp_d, int k, int sub_bits,
+                                                       const JobParams<T>* __restrict__ jobs,
+                                                       int32_t njobs, int k, int sub_bits,
                                                        K* __restrict__ keys,
                                                        uint32_t* __restrict__ vals) {
     const int64_t pos = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (pos >= n) return;
+    // sorted positions of a job are its own row range, so the job follows
+    // from the position
+    const JobParams<T>& jp = jobs[job_of_row(jobs, njobs, pos)];
     const int64_t row = inrow_of_pos[pos];
     uint32_t sub = 0, mul = 1;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         double v;
-        scaled<T>(pts[row * D + j], minv[j], rcp_f, rcp_d, &v);
+        scaled<T>(pts[row * D + j], jp.minv[j], jp.rcp_f, jp.rcp_d, &v);
         // v is the value the cell index floors (held exactly in double)
         const double c = floor(v);
-        long long s = static_cast<long long>(floor(v * k)) - static_cast<long long>(c) * k;
-        s = s < 0 ? 0 : (s >= k ? k - 1 : s);
-        sub += static_cast<uint32_t>(s) * mul;
+        long long sd = static_cast<long long>(floor(v * k)) - static_cast<long long>(c) * k;
+        sd = sd < 0 ? 0 : (sd >= k ? k - 1 : sd);
+        sub += static_cast<uint32_t>(sd) * mul;
         mul *= static_cast<uint32_t>(k);
     }
     const uint64_t u = static_cast<uint64_t>(scan[pos] + flags[pos] - 1u);  // occupied-cell index
@@ -219,24 +222,27 @@ __global__ __launch_bounds__(kThreads) void k_box_reduce(const T* __restrict__ s
 // tight flag per box: dist2(extent) <= EPS2 in the reference's order
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restrict__ bbox,
-                                                        int32_t nbox, T eps2,
+                                                        int32_t nbox,
                                                         const int32_t* __restrict__ bstart,
-                                                        uint32_t min_pts,
+                                                        const int32_t* __restrict__ bcell,
+                                                        const int32_t* __restrict__ job_of_cell,
+                                                        const JobParams<T>* __restrict__ jobs,
                                                         uint8_t* __restrict__ btight,
                                                         unsigned int* __restrict__ nloose) {
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox) return;
+    const JobParams<T>& jp = jobs[job_of_cell[bcell[b]]];
     T lo[D], hi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         lo[j] = ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2]);
         hi[j] = ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1]);
     }
-    const bool tight = within<D>(hi, lo, eps2);
+    const bool tight = within<D>(hi, lo, jp.eps2);
     const int32_t size = bstart[b + 1] - bstart[b];
     // bit 0: tight; bit 1: tight and at least minPts rows (all rows core)
     btight[b] = static_cast<uint8_t>((tight ? 1 : 0) |
-                                     ((tight && static_cast<uint32_t>(size) >= min_pts) ? 2 : 0));
+                                     ((tight && static_cast<uint32_t>(size) >= jp.min_pts) ? 2 : 0));
     if (!tight && size > 1) atomicAdd(nloose, 1u);
 }
 
@@ -250,9 +256,13 @@ __global__ __launch_bounds__(kThreads) void k_box_union(
     const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
     const uint32_t* __restrict__ roff, const int2* __restrict__ rcells,
     const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ bmincore,
-    const uint8_t* __restrict__ core, T eps2, double gap2_max, int32_t* bparent) {
+    const uint8_t* __restrict__ core, const int32_t* __restrict__ job_of_cell,
+    const JobParams<T>* __restrict__ jobs, int32_t* bparent) {
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox || bmincore[b] == INT_MAX) return;
+    const JobParams<T>& jp = jobs[job_of_cell[bcell[b]]];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
     double lo[D], hi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -375,10 +385,10 @@ int box_sub_bits(int d) {
 
 template <typename T, int D, typename K>
 void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
-                     const uint32_t* scan, const T* minv, float rcp_f, double rcp_d, K* keys,
+                     const uint32_t* scan, const JobParams<T>* jobs, int32_t njobs, K* keys,
                      uint32_t* vals, hipStream_t s) {
     hipLaunchKernelGGL((k_box_keys<T, D, K>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                       pts, n, inrow_of_pos, flags, scan, minv, rcp_f, rcp_d, box_split(D),
+                       pts, n, inrow_of_pos, flags, scan, jobs, njobs, box_split(D),
                        box_sub_bits(D), keys, vals);
     HPD_HIP(hipGetLastError());
 }
@@ -446,27 +456,25 @@ void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bminco
 }
 
 template <typename T, int D>
-void launch_box_tight(const BoxArgs<T>& a, T eps2, uint32_t min_pts, uint8_t* btight,
-                      unsigned int* nloose, hipStream_t s) {
+void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s) {
     using U = OrdKey<T>;
     hipLaunchKernelGGL((k_box_tight<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
-                       s, reinterpret_cast<const U*>(a.bbox), a.nbox, eps2, a.bstart, min_pts,
-                       btight, nloose);
+                       s, reinterpret_cast<const U*>(a.bbox), a.nbox, a.bstart, a.bcell,
+                       a.job_of_cell, a.jobs, btight, nloose);
     HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D>
 void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                      const int32_t* bmincore, const uint8_t* core, T eps2, int32_t* bparent,
+                      const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
                       int32_t* compmin, int32_t* bcluster, hipStream_t s) {
     using U = OrdKey<T>;
-    const double e2 = static_cast<double>(eps2);
-    const double gap2_max = e2 * (1.0 + 1e-5) + 1e-30;
     hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        bparent, static_cast<int64_t>(a.nbox));
     hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
                        s, a.soa, a.ld, a.nbox, a.bstart, a.bcell, a.cbox, roff, rcells,
-                       reinterpret_cast<const U*>(a.bbox), bmincore, core, eps2, gap2_max, bparent);
+                       reinterpret_cast<const U*>(a.bbox), bmincore, core, a.job_of_cell, a.jobs,
+                       bparent);
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        compmin, INT_MAX, static_cast<int64_t>(a.nbox));
     hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
@@ -478,21 +486,21 @@ void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rce
 
 #define HPD_INST_BOX_KEYS(T, D)                                                                   \
     template void launch_box_keys<T, D, uint32_t>(const T*, int64_t, const uint32_t*,             \
-                                                  const uint32_t*, const uint32_t*, const T*,     \
-                                                  float, double, uint32_t*, uint32_t*,            \
-                                                  hipStream_t);                                   \
+                                                  const uint32_t*, const uint32_t*,               \
+                                                  const JobParams<T>*, int32_t, uint32_t*,        \
+                                                  uint32_t*, hipStream_t);                        \
     template void launch_box_keys<T, D, uint64_t>(const T*, int64_t, const uint32_t*,             \
-                                                  const uint32_t*, const uint32_t*, const T*,     \
-                                                  float, double, uint64_t*, uint32_t*,            \
-                                                  hipStream_t);                                   \
+                                                  const uint32_t*, const uint32_t*,               \
+                                                  const JobParams<T>*, int32_t, uint64_t*,        \
+                                                  uint32_t*, hipStream_t);                        \
     template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, const uint32_t*,  \
                                             T*, int64_t, uint32_t*, hipStream_t);                 \
     template void launch_box_reduce<T, D>(const BoxArgs<T>&, const uint8_t*, int32_t*,            \
                                           hipStream_t);                                           \
-    template void launch_box_tight<T, D>(const BoxArgs<T>&, T, uint32_t, uint8_t*,               \
-                                         unsigned int*, hipStream_t);                             \
+    template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
+                                         hipStream_t);                                            \
     template void launch_box_union<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,         \
-                                         const int32_t*, const uint8_t*, T, int32_t*, int32_t*,   \
+                                         const int32_t*, const uint8_t*, int32_t*, int32_t*,      \
                                          int32_t*, hipStream_t);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \

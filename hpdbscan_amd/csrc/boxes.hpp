@@ -25,12 +25,14 @@ struct BoxArgs {
     int32_t nbitems = 0;
     const uint32_t* pos_of_slot = nullptr;
     void* bbox = nullptr;              // ordered-key min/max per box and dim
+    const JobParams<T>* jobs = nullptr;
+    const int32_t* job_of_cell = nullptr;
 };
 
 // key = (occupied-cell index << sub_bits) | sub-box digit, val = position
 template <typename T, int D, typename K>
 void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
-                     const uint32_t* scan, const T* minv, float rcp_f, double rcp_d, K* keys,
+                     const uint32_t* scan, const JobParams<T>* jobs, int32_t njobs, K* keys,
                      uint32_t* vals, hipStream_t s);
 
 template <typename T, int D>
@@ -62,14 +64,13 @@ void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bminco
 // btight[b]: bit 0 tight, bit 1 tight with >= minPts rows; *nloose counts
 // boxes of two or more rows that are not tight
 template <typename T, int D>
-void launch_box_tight(const BoxArgs<T>& a, T eps2, uint32_t min_pts, uint8_t* btight,
-                      unsigned int* nloose, hipStream_t s);
+void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s);
 
 // components of the boxes with cores; bcluster[b] = smallest core position
 // of b's component (-1 without cores)
 template <typename T, int D>
 void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                      const int32_t* bmincore, const uint8_t* core, T eps2, int32_t* bparent,
+                      const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
                       int32_t* compmin, int32_t* bcluster, hipStream_t s);
 
 }  // namespace hpd

@@ -240,7 +240,7 @@ static inline int bits_for(uint64_t max_value) {
 
 // worst-case device workspace of one run with n rows
 template <typename T, int D>
-static size_t workspace_bytes(int64_t n) {
+static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     const size_t nn = static_cast<size_t>(n);
     const size_t L = align_up(nn, 16);
     const size_t hist_words = radix_hist_words(n);
@@ -254,14 +254,19 @@ static size_t workspace_bytes(int64_t n) {
     b += L + 3 * 4 * L + 3 * 4 * nn;           // core, parent, rsnap, rootslot, cmin/cmax/rcenter
     b += 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // box tables
     b += 2 * D * 8 * nn;                       // box bounding boxes
-    b += sizeof(T) * 2 * D * 1024 + 40 * 256 + 64 * 1024;  // min/max partials, alignment
+    const size_t nj = static_cast<size_t>(njobs);
+    const size_t mm_blocks = nn / static_cast<size_t>(minmax_rows_per_block()) + nj + 1;
+    b += sizeof(T) * 2 * D * (mm_blocks + nj) + 4 * (mm_blocks + nj + 2) + 64;  // min/max
+    b += sizeof(JobParams<T>) * nj + 4 * nn;   // job table, job of every cell
+    if (one_thread) b += 8 * nn + 4 * nn + nn + 4;  // ambiguous rows, f, involved, flag
+    b += 52 * 256 + 64 * 1024;                 // alignment
     return b;
 }
 
 template <typename T, int D, typename K>
-static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t min_pts,
-                      int64_t* labels, hipStream_t s, const Geometry& g, float rcp_f,
-                      double rcp_d, const T* minv_dev, PhaseTimer& tm) {
+static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>* jobs,
+                      int32_t njobs, int key_bits, int64_t* labels, hipStream_t s,
+                      PhaseTimer& tm) {
     constexpr bool kFloat = std::is_floating_point<T>::value;
     Arena& A = ctx.main;
     const size_t nn = static_cast<size_t>(n);
@@ -287,25 +292,19 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     int32_t* cmin = A.take_n<int32_t>(nn);
     int32_t* cmax = A.take_n<int32_t>(nn);
     int32_t* rcenter = A.take_n<int32_t>(nn);
+    int32_t* job_of_cell = A.take_n<int32_t>(nn);
     unsigned long long* dstats = A.take_n<unsigned long long>(16);
     uint32_t* counters = reinterpret_cast<uint32_t*>(dstats + 8);  // m, ranges, items, boxes, box items, loose
     HPD_HIP(hipMemsetAsync(dstats, 0, 16 * sizeof(unsigned long long), s));
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.host_scratch);
 
-    CellParams cp{};
-    cp.d = D;
-    for (int k = 0; k < D; ++k) {
-        cp.swapped[k] = g.swapped[k];
-        cp.stride[k] = g.stride[k];
-        cp.ncell[k] = g.ncell[k];
-    }
-
-    // cell ids (spatial_index.h:144-165)
-    launch_cell_ids<T, D, K>(pts, n, minv_dev, cp, rcp_f, rcp_d, keys_a, vals_a, s);
+    // cell ids (spatial_index.h:144-165), offset by the job's key base
+    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, keys_a, vals_a, s);
     tm.mark();  // 1: grid done
 
-    // stable sort by cell (spatial_index.h:167-216): position -> input row
-    const bool in_b = radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, g.key_bits, hist, tmp, s);
+    // stable sort by cell (spatial_index.h:167-216): position -> input row;
+    // jobs stay contiguous and in order because their key ranges are
+    const bool in_b = radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, key_bits, hist, tmp, s);
     K* keys = in_b ? keys_b : keys_a;
     uint32_t* inrow_of_pos = in_b ? vals_b : vals_a;
 
@@ -348,16 +347,16 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
         if (bits2 <= 32) {
             uint32_t* ka = reinterpret_cast<uint32_t*>(k2a);
             uint32_t* kb = reinterpret_cast<uint32_t*>(k2b);
-            launch_box_keys<T, D, uint32_t>(pts, n, inrow_of_pos, flags, flags + n, minv_dev, rcp_f,
-                                            rcp_d, ka, v2a, s);
+            launch_box_keys<T, D, uint32_t>(pts, n, inrow_of_pos, flags, flags + n, jobs, njobs,
+                                            ka, v2a, s);
             const bool ib = radix_sort_pairs<uint32_t>(ka, v2a, kb, v2b, n, bits2, hist, tmp, s);
             slot_pos = ib ? v2b : v2a;
             sorted2 = ib ? static_cast<const void*>(kb) : static_cast<const void*>(ka);
             launch_box_table<uint32_t>(static_cast<const uint32_t*>(sorted2), n, sub_bits, flags, tmp,
                                        bstart, bcell, box_of_slot, &counters[3], s);
         } else {
-            launch_box_keys<T, D, uint64_t>(pts, n, inrow_of_pos, flags, flags + n, minv_dev, rcp_f,
-                                            rcp_d, k2a, v2a, s);
+            launch_box_keys<T, D, uint64_t>(pts, n, inrow_of_pos, flags, flags + n, jobs, njobs,
+                                            k2a, v2a, s);
             const bool ib = radix_sort_pairs<uint64_t>(k2a, v2a, k2b, v2b, n, bits2, hist, tmp, s);
             slot_pos = ib ? v2b : v2a;
             sorted2 = ib ? static_cast<const void*>(k2b) : static_cast<const void*>(k2a);
@@ -372,7 +371,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     tm.mark();  // 2: sort done
 
     // neighbour ranges (spatial_index.h:682-737) and work items
-    launch_neighbor_count<K>(ucell, ustart, m, cp, nranges, &dstats[0], s);
+    launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, &dstats[0], s);
     exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
     exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
@@ -398,7 +397,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     int2* rcells = ctx.aux.take_n<int2>(total_ranges);
     int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
     int2* bitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
-    launch_neighbor_fill<K>(ucell, ustart, m, cp, nranges, ranges, rcells, rcenter, s);
+    launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
+                               rcenter, s);
     launch_work_items(ustart, m, nchunks, items, s);
 
     DbscanArgs<T> a;
@@ -413,8 +413,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     a.rcells = rcells;
     a.rcenter = rcenter;
     a.m = m;
-    a.eps2 = eps2_as<T>(eps);
-    a.min_pts = static_cast<uint32_t>(std::min<uint64_t>(min_pts, 0xffffffffull));
+    a.jobs = jobs;
+    a.job_of_cell = job_of_cell;
     a.pos_of_slot = pos_of_slot;
     a.inrow = inrow_of_slot;
 
@@ -433,8 +433,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
         ba.nbitems = nbitems;
         ba.pos_of_slot = pos_of_slot;
         ba.bbox = bbox;
+        ba.jobs = jobs;
+        ba.job_of_cell = job_of_cell;
         launch_box_reduce<T, D>(ba, nullptr, nullptr, s);
-        launch_box_tight<T, D>(ba, a.eps2, a.min_pts, btight, &counters[5], s);
+        launch_box_tight<T, D>(ba, btight, &counters[5], s);
         a.box_of_slot = box_of_slot;
         a.btight = btight;
     }
@@ -450,20 +452,42 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     }
     if (box_union) {
         launch_box_reduce<T, D>(ba, core, bmincore, s);
-        launch_box_union<T, D>(ba, nranges, rcells, bmincore, core, a.eps2, bparent, compmin,
-                               bcluster, s);
+        launch_box_union<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, bcluster, s);
     } else {
         if (!compmin) compmin = A.take_n<int32_t>(nn);
         launch_union_points<T, D>(a, core, parent, rsnap, cmin, cmax, compmin, s);
     }
     tm.mark();  // 5: union done
+    const bool one_thread = ctx.border_rule == HPD_BORDER_ONE_THREAD;
+    int2* amb = one_thread ? A.take_n<int2>(nn) : nullptr;
     launch_labels<T, D>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
-                        cmax, labels, &dstats[2], s);
-    tm.mark();  // 6: labels done
+                        cmax, labels, &dstats[2], amb, s);
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
     HPD_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
+    if (one_thread && hs[3] > 0) {
+        // the reference's choice at one thread for the border rows next to
+        // two or more clusters (hpdbscan.h:74-92)
+        const int32_t namb = static_cast<int32_t>(hs[3]);
+        int32_t* f = A.take_n<int32_t>(nn);
+        uint8_t* involved = A.take_n<uint8_t>(nn);
+        unsigned int* changed = A.take_n<unsigned int>(1);
+        HPD_HIP(hipMemsetAsync(involved, 0, nn, s));
+        launch_one_thread_mark<T, D>(a, core, rootslot, amb, namb, involved, f, s);
+        // sweep k leaves every core whose best decreasing path has at most
+        // k steps final, so sweep n + 1 at the latest changes nothing
+        for (int64_t sweep = 0;; ++sweep) {
+            if (sweep > n + 1) throw Error(HPD_EDEVICE, "border relaxation did not converge");
+            HPD_HIP(hipMemsetAsync(changed, 0, sizeof(unsigned int), s));
+            launch_one_thread_relax<T, D>(a, core, rootslot, involved, f, changed, s);
+            HPD_HIP(hipMemcpyAsync(hc + 32, changed, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+            HPD_HIP(hipStreamSynchronize(s));
+            if (hc[32] == 0) break;
+        }
+        launch_one_thread_assign<T, D>(a, core, rootslot, amb, namb, f, labels, s);
+    }
+    tm.mark();  // 6: labels done
     ctx.stats.sum_candidates = hs[0];
     ctx.stats.core_points = hs[1];
     ctx.stats.noise_points = hs[2];
@@ -473,41 +497,93 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, float eps, uint64_t
     ctx.stats.box_union = box_union ? 1 : 0;
 }
 
+// A batch of jobs: rows [offsets[j], offsets[j+1]) of `pts` form job j with
+// its own eps[j] and min_pts[j].  A plain call is a batch of one.
 template <typename T, int D>
-static void run_dims(Context& ctx, const T* pts, int64_t n, float eps, uint64_t min_pts,
-                     int64_t* labels, hipStream_t s) {
+static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offsets,
+                      int32_t njobs, const float* eps, const uint64_t* min_pts, int64_t* labels,
+                      hipStream_t s) {
     PhaseTimer tm(ctx, s);
     tm.mark();  // 0: start
     Arena& A = ctx.main;
-    A.reserve(workspace_bytes<T, D>(n));
+    A.reserve(workspace_bytes<T, D>(n, njobs, ctx.border_rule == HPD_BORDER_ONE_THREAD));
 
-    // min / max (spatial_index.h:99-122)
-    const int nb = minmax_partial_blocks(n);
-    T* part = A.take_n<T>(static_cast<size_t>(nb) * 2 * D);
-    T* mm = A.take_n<T>(2 * D + 8);
-    int* nonfinite = reinterpret_cast<int*>(mm + 2 * D);
+    // job table, first pass: row ranges and min/max blocks.  The host
+    // vectors outlive every copy from them: the run ends with a stream sync.
+    std::vector<JobParams<T>> hj(static_cast<size_t>(njobs));
+    std::vector<int32_t> bfirst(static_cast<size_t>(njobs) + 1, 0);
+    const int rpb = minmax_rows_per_block();
+    int32_t nblocks = 0;
+    for (int32_t j = 0; j < njobs; ++j) {
+        hj[j] = JobParams<T>{};
+        hj[j].row_begin = offsets[j];
+        hj[j].row_end = offsets[j + 1];
+        bfirst[j] = nblocks;
+        nblocks += static_cast<int32_t>((offsets[j + 1] - offsets[j] + rpb - 1) / rpb);
+    }
+    bfirst[njobs] = nblocks;
+    JobParams<T>* jobs_dev = A.take_n<JobParams<T>>(static_cast<size_t>(njobs));
+    int32_t* bfirst_dev = A.take_n<int32_t>(static_cast<size_t>(njobs) + 1);
+    T* part = A.take_n<T>(static_cast<size_t>(nblocks) * 2 * D + 1);
+    T* mm = A.take_n<T>(static_cast<size_t>(njobs) * 2 * D + 16);
+    int* nonfinite = reinterpret_cast<int*>(A.take_n<int>(4));
+    HPD_HIP(hipMemcpyAsync(jobs_dev, hj.data(), sizeof(JobParams<T>) * njobs, hipMemcpyHostToDevice, s));
+    HPD_HIP(hipMemcpyAsync(bfirst_dev, bfirst.data(), sizeof(int32_t) * (njobs + 1),
+                           hipMemcpyHostToDevice, s));
     HPD_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
-    launch_minmax<T, D>(pts, n, part, mm, nonfinite, s);
-    T hmm[2 * D];
-    int hbad = 0;
-    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, mm, sizeof(hmm) + sizeof(int), hipMemcpyDeviceToHost, s));
-    HPD_HIP(hipStreamSynchronize(s));
-    std::memcpy(hmm, ctx.host_scratch, sizeof(hmm));
-    std::memcpy(&hbad, reinterpret_cast<const char*>(ctx.host_scratch) + sizeof(hmm), sizeof(int));
-    if (hbad) throw Error(HPD_EINVAL, "input contains NaN or infinite coordinates");
-    float rcp_f;
-    double rcp_d;
-    const Geometry g = make_geometry<T>(hmm, hmm + D, D, eps, &rcp_f, &rcp_d);
-    if (g.overflow)
-        throw Error(HPD_EUNSUPPORTED,
-                    "the eps-grid has more than 2^64 cells (the reference's cell ids wrap)");
-    ctx.stats.total_cells = g.total;
-    ctx.stats.key_bits = g.key_bits;
 
-    if (g.key_bits <= 32)
-        run_keyed<T, D, uint32_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm);
+    // min / max per job (spatial_index.h:99-122)
+    launch_minmax<T, D>(pts, jobs_dev, njobs, bfirst_dev, nblocks, part, mm, nonfinite, s);
+    std::vector<T> hmm(static_cast<size_t>(njobs) * 2 * D);
+    int hbad = 0;
+    HPD_HIP(hipMemcpyAsync(hmm.data(), mm, sizeof(T) * njobs * 2 * D, hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(ctx.host_scratch, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipStreamSynchronize(s));
+    std::memcpy(&hbad, ctx.host_scratch, sizeof(int));
+    if (hbad) throw Error(HPD_EINVAL, "input contains NaN or infinite coordinates");
+
+    // per-job grid (spatial_index.h:124-142) and key ranges
+    uint64_t key_total = 0;
+    for (int32_t j = 0; j < njobs; ++j) {
+        JobParams<T>& jp = hj[j];
+        jp.key_base = key_total;
+        jp.min_pts = static_cast<uint32_t>(std::min<uint64_t>(min_pts[j], 0xffffffffull));
+        jp.eps2 = eps2_as<T>(eps[j]);
+        jp.gap2_max = static_cast<double>(jp.eps2) * (1.0 + 1e-5) + 1e-30;
+        jp.cp.d = D;
+        if (jp.row_end == jp.row_begin) continue;
+        float rcp_f;
+        double rcp_d;
+        const T* mn = &hmm[static_cast<size_t>(j) * 2 * D];
+        const Geometry g = make_geometry<T>(mn, mn + D, D, eps[j], &rcp_f, &rcp_d);
+        if (g.overflow)
+            throw Error(HPD_EUNSUPPORTED,
+                        "the eps-grid has more than 2^64 cells (the reference's cell ids wrap)");
+        jp.rcp_f = rcp_f;
+        jp.rcp_d = rcp_d;
+        for (int k = 0; k < D; ++k) {
+            jp.minv[k] = mn[k];
+            jp.cp.swapped[k] = g.swapped[k];
+            jp.cp.stride[k] = g.stride[k];
+            jp.cp.ncell[k] = g.ncell[k];
+        }
+        uint64_t t;
+        if (__builtin_add_overflow(key_total, g.total, &t))
+            throw Error(HPD_EUNSUPPORTED, "the jobs' grids have more than 2^64 cells together");
+        key_total = t;
+        if (njobs == 1) {
+            ctx.stats.total_cells = g.total;
+        }
+    }
+    if (njobs != 1) ctx.stats.total_cells = key_total;
+    const int key_bits = bits_for(key_total > 0 ? key_total - 1 : 0);
+    ctx.stats.key_bits = key_bits;
+    HPD_HIP(hipMemcpyAsync(jobs_dev, hj.data(), sizeof(JobParams<T>) * njobs, hipMemcpyHostToDevice, s));
+
+    if (key_bits <= 32)
+        run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, labels, s, tm);
     else
-        run_keyed<T, D, uint64_t>(ctx, pts, n, eps, min_pts, labels, s, g, rcp_f, rcp_d, mm, tm);
+        run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, labels, s, tm);
     if (ctx.timing) {
         ctx.stats.ms_grid = tm.ms(0, 1);
         ctx.stats.ms_sort = tm.ms(1, 2);
@@ -522,17 +598,18 @@ static void run_dims(Context& ctx, const T* pts, int64_t n, float eps, uint64_t 
 }
 
 template <typename T>
-static void run_typed(Context& ctx, const T* pts, int64_t n, int d, float eps, uint64_t min_pts,
-                      int64_t* labels, hipStream_t s) {
+static void run_typed(Context& ctx, const T* pts, int64_t n, int d, const int64_t* offsets,
+                      int32_t njobs, const float* eps, const uint64_t* min_pts, int64_t* labels,
+                      hipStream_t s) {
     switch (d) {
-        case 1: return run_dims<T, 1>(ctx, pts, n, eps, min_pts, labels, s);
-        case 2: return run_dims<T, 2>(ctx, pts, n, eps, min_pts, labels, s);
-        case 3: return run_dims<T, 3>(ctx, pts, n, eps, min_pts, labels, s);
-        case 4: return run_dims<T, 4>(ctx, pts, n, eps, min_pts, labels, s);
-        case 5: return run_dims<T, 5>(ctx, pts, n, eps, min_pts, labels, s);
-        case 6: return run_dims<T, 6>(ctx, pts, n, eps, min_pts, labels, s);
-        case 7: return run_dims<T, 7>(ctx, pts, n, eps, min_pts, labels, s);
-        case 8: return run_dims<T, 8>(ctx, pts, n, eps, min_pts, labels, s);
+        case 1: return run_batch<T, 1>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 2: return run_batch<T, 2>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 3: return run_batch<T, 3>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 4: return run_batch<T, 4>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 5: return run_batch<T, 5>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 6: return run_batch<T, 6>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 7: return run_batch<T, 7>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
+        case 8: return run_batch<T, 8>(ctx, pts, n, offsets, njobs, eps, min_pts, labels, s);
         default:
             throw Error(HPD_EUNSUPPORTED, "dimensions above " + std::to_string(kMaxDims) +
                                               " are not supported by this build");
@@ -551,21 +628,27 @@ static size_t dtype_size(int dt) {
     }
 }
 
-static void run_any(Context& ctx, int dt, const void* pts, int64_t n, int d, float eps,
-                    uint64_t min_pts, int64_t* labels, hipStream_t s) {
+static void run_any(Context& ctx, int dt, const void* pts, int64_t n, int d,
+                    const int64_t* offsets, int32_t njobs, const float* eps,
+                    const uint64_t* min_pts, int64_t* labels, hipStream_t s) {
+#define HPD_RUN_CASE(CODE, T)                                                                \
+    case CODE:                                                                               \
+        return run_typed(ctx, static_cast<const T*>(pts), n, d, offsets, njobs, eps, min_pts, \
+                         labels, s);
     switch (dt) {
-        case HPD_DTYPE_F32: return run_typed(ctx, static_cast<const float*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_F64: return run_typed(ctx, static_cast<const double*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_I8: return run_typed(ctx, static_cast<const int8_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_I16: return run_typed(ctx, static_cast<const int16_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_I32: return run_typed(ctx, static_cast<const int32_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_I64: return run_typed(ctx, static_cast<const int64_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_U8: return run_typed(ctx, static_cast<const uint8_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_U16: return run_typed(ctx, static_cast<const uint16_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_U32: return run_typed(ctx, static_cast<const uint32_t*>(pts), n, d, eps, min_pts, labels, s);
-        case HPD_DTYPE_U64: return run_typed(ctx, static_cast<const uint64_t*>(pts), n, d, eps, min_pts, labels, s);
+        HPD_RUN_CASE(HPD_DTYPE_F32, float)
+        HPD_RUN_CASE(HPD_DTYPE_F64, double)
+        HPD_RUN_CASE(HPD_DTYPE_I8, int8_t)
+        HPD_RUN_CASE(HPD_DTYPE_I16, int16_t)
+        HPD_RUN_CASE(HPD_DTYPE_I32, int32_t)
+        HPD_RUN_CASE(HPD_DTYPE_I64, int64_t)
+        HPD_RUN_CASE(HPD_DTYPE_U8, uint8_t)
+        HPD_RUN_CASE(HPD_DTYPE_U16, uint16_t)
+        HPD_RUN_CASE(HPD_DTYPE_U32, uint32_t)
+        HPD_RUN_CASE(HPD_DTYPE_U64, uint64_t)
         default: throw Error(HPD_EINVAL, "unknown dtype code " + std::to_string(dt));
     }
+#undef HPD_RUN_CASE
 }
 
 static void validate(const Context* ctx, int dt, const void* data, int64_t n, int32_t d, float eps,
@@ -664,8 +747,6 @@ int hpd_context_set_border_rule(hpd_context* ctx, int rule) {
         if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
         if (rule != HPD_BORDER_MIN_CLUSTER && rule != HPD_BORDER_ONE_THREAD)
             throw Error(HPD_EINVAL, "unknown border rule");
-        if (rule == HPD_BORDER_ONE_THREAD)
-            throw Error(HPD_EUNSUPPORTED, "the one-thread border rule is not available yet");
         ctx->impl.border_rule = rule;
     });
 }
@@ -695,7 +776,8 @@ int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n,
         if (n == 0) return;
         hpd::DeviceGuard g(c.device);
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
-        hpd::run_any(c, dtype, data, n, d, eps, min_pts, labels_out, s);
+        const int64_t offsets[2] = {0, n};
+        hpd::run_any(c, dtype, data, n, d, offsets, 1, &eps, &min_pts, labels_out, s);
     });
 }
 
@@ -717,7 +799,65 @@ int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_
         void* din = c.io.take(in_bytes);
         int64_t* dout = c.io.take_n<int64_t>(static_cast<size_t>(n));
         HPD_HIP(hipMemcpyAsync(din, data, in_bytes, hipMemcpyHostToDevice, c.stream));
-        hpd::run_any(c, dtype, din, n, d, eps, min_pts, dout, c.stream);
+        const int64_t offsets[2] = {0, n};
+        hpd::run_any(c, dtype, din, n, d, offsets, 1, &eps, &min_pts, dout, c.stream);
+        HPD_HIP(hipMemcpyAsync(labels_out, dout, out_bytes, hipMemcpyDeviceToHost, c.stream));
+        HPD_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+static void validate_batch(const hpd_context* ctx, int dtype, const void* data,
+                           const int64_t* offsets, int32_t njobs, int32_t d, const float* eps,
+                           const uint64_t* min_pts, const void* labels) {
+    if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+    if (njobs < 0) throw Error(HPD_EINVAL, "the number of jobs must not be negative");
+    if (njobs == 0) return;
+    if (!offsets || !eps || !min_pts) throw Error(HPD_EINVAL, "offsets, eps and min_pts must not be NULL");
+    if (offsets[0] != 0) throw Error(HPD_EINVAL, "offsets[0] must be 0");
+    for (int32_t j = 0; j < njobs; ++j) {
+        if (offsets[j + 1] < offsets[j]) throw Error(HPD_EINVAL, "offsets must not decrease");
+        if (!(eps[j] > 0.0f))
+            throw Error(HPD_EINVAL, "epsilon needs to be positive (job " + std::to_string(j) + ")");
+    }
+    hpd::validate(&ctx->impl, dtype, data, offsets[njobs], d, 1.0f, labels);
+}
+
+int hpd_cluster_batch_device(hpd_context* ctx, int dtype, const void* data,
+                             const int64_t* offsets, int32_t njobs, int32_t d, const float* eps,
+                             const uint64_t* min_pts, int64_t* labels_out, void* stream) {
+    return guarded([&] {
+        validate_batch(ctx, dtype, data, offsets, njobs, d, eps, min_pts, labels_out);
+        Context& c = ctx->impl;
+        c.stats = hpd_run_stats{};
+        const int64_t n = njobs ? offsets[njobs] : 0;
+        c.stats.n = n;
+        c.stats.d = d;
+        if (n == 0) return;
+        hpd::DeviceGuard g(c.device);
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
+        hpd::run_any(c, dtype, data, n, d, offsets, njobs, eps, min_pts, labels_out, s);
+    });
+}
+
+int hpd_cluster_batch(hpd_context* ctx, int dtype, const void* data, const int64_t* offsets,
+                      int32_t njobs, int32_t d, const float* eps, const uint64_t* min_pts,
+                      int64_t* labels_out) {
+    return guarded([&] {
+        validate_batch(ctx, dtype, data, offsets, njobs, d, eps, min_pts, labels_out);
+        Context& c = ctx->impl;
+        c.stats = hpd_run_stats{};
+        const int64_t n = njobs ? offsets[njobs] : 0;
+        c.stats.n = n;
+        c.stats.d = d;
+        if (n == 0) return;
+        hpd::DeviceGuard g(c.device);
+        const size_t in_bytes = hpd::dtype_size(dtype) * static_cast<size_t>(d) * static_cast<size_t>(n);
+        const size_t out_bytes = sizeof(int64_t) * static_cast<size_t>(n);
+        c.io.reserve(hpd::align_up(in_bytes, 256) + out_bytes + 512);
+        void* din = c.io.take(in_bytes);
+        int64_t* dout = c.io.take_n<int64_t>(static_cast<size_t>(n));
+        HPD_HIP(hipMemcpyAsync(din, data, in_bytes, hipMemcpyHostToDevice, c.stream));
+        hpd::run_any(c, dtype, din, n, d, offsets, njobs, eps, min_pts, dout, c.stream);
         HPD_HIP(hipMemcpyAsync(labels_out, dout, out_bytes, hipMemcpyDeviceToHost, c.stream));
         HPD_HIP(hipStreamSynchronize(c.stream));
     });

@@ -151,17 +151,20 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
                                                    unsigned long long* __restrict__ ncore) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
+    const T eps2 = jp.eps2;
+    const uint32_t min_pts = jp.min_pts;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     T pc[D];
     load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
     uint32_t cnt = 0;
     // rows of a tight box with >= minPts rows are core without counting
-    if (a.btight && valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = a.min_pts;
+    if (a.btight && valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = min_pts;
     auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
-        cnt += within<D>(pc, qc, a.eps2) ? 1u : 0u;
+        cnt += within<D>(pc, qc, eps2) ? 1u : 0u;
     };
-    auto stop = [&] { return static_cast<bool>(__all(!valid || cnt >= a.min_pts)); };
+    auto stop = [&] { return static_cast<bool>(__all(!valid || cnt >= min_pts)); };
     bool done = stop();
     // the cell's own row first: most neighbours are there
     const int32_t center = uniform(a.rcenter[c.u]);
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
         done = visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, uniform(rg.x), uniform(rg.y), body,
                                  stop);
     }
-    const bool is_core = valid && cnt >= a.min_pts;
+    const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
     const unsigned long long nc = __popcll(ballot(is_core));
     if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
@@ -184,19 +187,20 @@ __global__ __launch_bounds__(kThreads) void k_hook(const DbscanArgs<T> a,
                                                    int32_t* __restrict__ parent) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const T eps2 = a.jobs[uniform(a.job_of_cell[c.u])].eps2;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     const bool active = valid && core[p];
     if (!__any(active)) return;
     T pc[D];
     load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
-    // candidates arrive in ascending position, so the first hit is the
+    // candidates arrive in ascending slot order, so the first hit is the
     // smallest adjacent core before p
     int32_t best = p;
     bool found = !active;
     auto body = [&](int32_t q, const T* qc, uint8_t cq, int32_t) {
         if (!cq) return;
-        if (!found && q < p && within<D>(pc, qc, a.eps2)) {
+        if (!found && q < p && within<D>(pc, qc, eps2)) {
             best = q;
             found = true;
         }
@@ -256,9 +260,11 @@ __global__ __launch_bounds__(kThreads) void k_compmin(const uint8_t* __restrict_
 
 // cluster of every core slot -> label, slot array of clusters, per-cell
 // smallest / largest cluster
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_label_cores(
     const int2* __restrict__ items, int32_t nitems, const int32_t* __restrict__ ustart,
-    const uint32_t* __restrict__ roff, const uint8_t* __restrict__ core,
+    const uint32_t* __restrict__ roff, const JobParams<T>* __restrict__ jobs,
+    const int32_t* __restrict__ job_of_cell, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ box_of_slot, const int32_t* __restrict__ bcluster, int32_t* parent,
     const int32_t* __restrict__ compmin, const uint32_t* __restrict__ inrow,
     int64_t* __restrict__ labels, int32_t* __restrict__ rootslot, int32_t* __restrict__ cmin,
@@ -269,10 +275,11 @@ __global__ __launch_bounds__(kThreads) void k_label_cores(
     const bool valid = p < c.end;
     const bool active = valid && core[p];
     if (!__any(active)) return;
+    const int64_t first_pos = jobs[uniform(job_of_cell[c.u])].row_begin;
     int32_t r = INT_MAX;
     if (active) {
         r = bcluster ? bcluster[box_of_slot[p]] : compmin[uf_find(parent, p)];
-        labels[inrow[p]] = -(static_cast<int64_t>(r) + 1);
+        labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
         rootslot[p] = r;
     }
     const int32_t rmin = wave_min(active ? r : INT_MAX);
@@ -292,6 +299,7 @@ __global__ __launch_bounds__(kThreads) void k_union(const DbscanArgs<T> a,
                                                     int32_t* parent) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const T eps2 = a.jobs[uniform(a.job_of_cell[c.u])].eps2;
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     const bool active = valid && core[p];
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_union(const DbscanArgs<T> a,
         if (!cq) return;
         const bool need = active && q < p && rq != rp;
         if (!__any(need)) return;
-        if (need && within<D>(pc, qc, a.eps2)) uf_unite(parent, p, q);
+        if (need && within<D>(pc, qc, eps2)) uf_unite(parent, p, q);
     };
     auto never = [] { return false; };
     for (int32_t r = c.r0; r < c.r1; ++r) {
@@ -328,9 +336,13 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
                                                      const int32_t* __restrict__ cmax,
                                                      const uint32_t* __restrict__ order,
                                                      int64_t* __restrict__ labels,
-                                                     unsigned long long* __restrict__ stats) {
+                                                     unsigned long long* __restrict__ stats,
+                                                     int2* __restrict__ amb_list) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
+    const T eps2 = jp.eps2;
+    const int64_t first_pos = jp.row_begin;  // cluster ids are positions inside the job
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     const bool active = valid && !core[p];
@@ -357,7 +369,7 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
         load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
         auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t rq) {
             if (!cq) return;
-            if (active && within<D>(pc, qc, a.eps2)) {
+            if (active && within<D>(pc, qc, eps2)) {
                 best = min(best, rq);
                 worst = max(worst, rq);
             }
@@ -375,13 +387,141 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
                 break;
         }
     }
-    if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) + 1;
+    if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
+    const bool is_amb = active && best != INT_MAX && best != worst;
     const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
-    const unsigned long long amb = __popcll(ballot(active && best != INT_MAX && best != worst));
+    const uint64_t amb_mask = ballot(is_amb);
+    const unsigned long long amb = __popcll(amb_mask);
+    unsigned long long base = 0;
     if (lane_id() == 0) {
         if (noise) atomicAdd(&stats[0], noise);
-        if (amb) atomicAdd(&stats[1], amb);
+        if (amb) base = atomicAdd(&stats[1], amb);
     }
+    // record (slot, cell) of every border row next to two or more clusters
+    if (amb_list && amb) {
+        base = __shfl(base, 0, 64);
+        if (is_amb)
+            amb_list[base + static_cast<unsigned long long>(__popcll(amb_mask & lanemask_lt()))] =
+                make_int2(p, c.u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The reference's border choice at one thread (HPD_BORDER_ONE_THREAD).
+//
+// Processing rows in sorted order, the reference gives core c the
+// provisional label f(c) + 1 with f(c) = min(pos(c), f(q) over within-eps
+// cores q before c) (hpdbscan.h:74-92 with spatial_index.h:757-763), i.e. the
+// smallest position reachable from c along strictly decreasing positions.
+// A border row keeps the smallest provisional label among its adjacent cores
+// (the atomic_min of hpdbscan.h:91) and ends in that core's cluster.  Only
+// border rows next to two or more clusters depend on it; f is computed for
+// the cores of the clusters they touch, by in-place relaxation to the fixed
+// point.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t slot_pos(const uint32_t* __restrict__ pos_of_slot, int32_t s) {
+    return pos_of_slot ? static_cast<int32_t>(pos_of_slot[s]) : s;
+}
+
+// clusters touched by the ambiguous border rows -> involved[cluster] = 1
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_ot_mark(const DbscanArgs<T> a,
+                                                      const uint8_t* __restrict__ core,
+                                                      const int32_t* __restrict__ rootslot,
+                                                      const int2* __restrict__ amb, int32_t namb,
+                                                      uint8_t* __restrict__ involved) {
+    const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= namb) return;
+    const int32_t p = amb[i].x, u = amb[i].y;
+    const T eps2 = a.jobs[a.job_of_cell[u]].eps2;
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, p, pc);
+    for (uint32_t r = a.roff[u]; r < a.roff[u + 1]; ++r) {
+        const int2 rg = a.ranges[r];
+        for (int32_t q = rg.x; q < rg.y; ++q) {
+            if (!core[q]) continue;
+            T qc[D];
+            load_row<T, D>(a.soa, a.ld, q, qc);
+            if (within<D>(pc, qc, eps2)) involved[rootslot[q]] = 1;
+        }
+    }
+}
+
+// one in-place relaxation sweep of f over the cores of involved clusters;
+// f is indexed by slot and holds positions
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_ot_relax(const DbscanArgs<T> a,
+                                                       const uint8_t* __restrict__ core,
+                                                       const int32_t* __restrict__ rootslot,
+                                                       const uint8_t* __restrict__ involved,
+                                                       int32_t* f, unsigned int* __restrict__ changed) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    const bool active = valid && core[p] && involved[rootslot[p]];
+    if (!__any(active)) return;
+    const T eps2 = a.jobs[uniform(a.job_of_cell[c.u])].eps2;
+    const int32_t pp = valid ? slot_pos(a.pos_of_slot, p) : 0;
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+    int32_t fmin = active ? ld_agent(&f[p]) : INT_MAX;
+    const int32_t f0 = fmin;
+    for (int32_t r = c.r0; r < c.r1; ++r) {
+        const int2 rg = a.ranges[r];
+        const int32_t qb = uniform(rg.x), qe = uniform(rg.y);
+        if (qb >= c.end) break;  // later cells hold later positions only
+        for (int32_t q = qb; q < qe; ++q) {
+            if (!core[q]) continue;
+            const int32_t qp = slot_pos(a.pos_of_slot, q);
+            T qc[D];
+            load_row<T, D>(a.soa, a.ld, q, qc);
+            if (active && qp < pp && within<D>(pc, qc, eps2)) fmin = min(fmin, ld_agent(&f[q]));
+        }
+    }
+    if (active && fmin < f0) {
+        st_agent(&f[p], fmin);
+        atomicOr(changed, 1u);
+    }
+}
+
+// final choice for every ambiguous border row: the cluster of the adjacent
+// core with the smallest f
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_ot_assign(const DbscanArgs<T> a,
+                                                        const uint8_t* __restrict__ core,
+                                                        const int32_t* __restrict__ rootslot,
+                                                        const int2* __restrict__ amb, int32_t namb,
+                                                        const int32_t* __restrict__ f,
+                                                        int64_t* __restrict__ labels) {
+    const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= namb) return;
+    const int32_t p = amb[i].x, u = amb[i].y;
+    const JobParams<T>& jp = a.jobs[a.job_of_cell[u]];
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, p, pc);
+    int32_t best_f = INT_MAX, best_cluster = -1;
+    for (uint32_t r = a.roff[u]; r < a.roff[u + 1]; ++r) {
+        const int2 rg = a.ranges[r];
+        for (int32_t q = rg.x; q < rg.y; ++q) {
+            if (!core[q]) continue;
+            T qc[D];
+            load_row<T, D>(a.soa, a.ld, q, qc);
+            if (within<D>(pc, qc, jp.eps2) && f[q] < best_f) {
+                best_f = f[q];
+                best_cluster = rootslot[q];
+            }
+        }
+    }
+    if (best_cluster >= 0)
+        labels[a.inrow[p]] = static_cast<int64_t>(best_cluster) - jp.row_begin + 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ot_init(const uint8_t* __restrict__ core,
+                                                      const uint32_t* __restrict__ pos_of_slot,
+                                                      int32_t* __restrict__ f, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) f[i] = core[i] ? slot_pos(pos_of_slot, static_cast<int32_t>(i)) : INT_MAX;
 }
 
 __global__ __launch_bounds__(kThreads) void k_iota(int32_t* __restrict__ v, int64_t n) {
@@ -435,17 +575,47 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 template <typename T, int D>
 void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                    int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                   int32_t* cmax, int64_t* labels, unsigned long long* stats, hipStream_t s) {
+                   int32_t* cmax, int64_t* labels, unsigned long long* stats, int2* amb_list,
+                   hipStream_t s) {
     if (a.n <= 0) return;
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
                        cmax, -1, static_cast<int64_t>(a.m));
-    hipLaunchKernelGGL(k_label_cores, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems, a.ustart,
-                       a.roff, core, a.box_of_slot, bcluster, parent, compmin, a.inrow, labels,
-                       rootslot, cmin, cmax);
+    hipLaunchKernelGGL(k_label_cores<T>, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems,
+                       a.ustart, a.roff, a.jobs, a.job_of_cell, core, a.box_of_slot, bcluster,
+                       parent, compmin, a.inrow, labels, rootslot, cmin, cmax);
     hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
                        static_cast<const int32_t*>(rootslot), static_cast<const int32_t*>(cmin),
-                       static_cast<const int32_t*>(cmax), a.inrow, labels, stats);
+                       static_cast<const int32_t*>(cmax), a.inrow, labels, stats, amb_list);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_one_thread_mark(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                            const int2* amb, int32_t namb, uint8_t* involved, int32_t* f,
+                            hipStream_t s) {
+    hipLaunchKernelGGL((k_ot_mark<T, D>), dim3(blocks_for(namb, kThreads)), dim3(kThreads), 0, s,
+                       a, core, rootslot, amb, namb, involved);
+    hipLaunchKernelGGL(k_ot_init, dim3(blocks_for(a.n, kThreads)), dim3(kThreads), 0, s, core,
+                       a.pos_of_slot, f, a.n);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_one_thread_relax(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                             const uint8_t* involved, int32_t* f, unsigned int* changed,
+                             hipStream_t s) {
+    hipLaunchKernelGGL((k_ot_relax<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a, core, rootslot, involved, f, changed);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                              const int2* amb, int32_t namb, const int32_t* f, int64_t* labels,
+                              hipStream_t s) {
+    hipLaunchKernelGGL((k_ot_assign<T, D>), dim3(blocks_for(namb, kThreads)), dim3(kThreads), 0, s,
+                       a, core, rootslot, amb, namb, f, labels);
     HPD_HIP(hipGetLastError());
 }
 
@@ -456,7 +626,16 @@ void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
                                             int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t); \
     template void launch_labels<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \
                                       int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*,     \
-                                      int64_t*, unsigned long long*, hipStream_t);
+                                      int64_t*, unsigned long long*, int2*, hipStream_t);         \
+    template void launch_one_thread_mark<T, D>(const DbscanArgs<T>&, const uint8_t*,              \
+                                               const int32_t*, const int2*, int32_t, uint8_t*,    \
+                                               int32_t*, hipStream_t);                            \
+    template void launch_one_thread_relax<T, D>(const DbscanArgs<T>&, const uint8_t*,             \
+                                                const int32_t*, const uint8_t*, int32_t*,         \
+                                                unsigned int*, hipStream_t);                      \
+    template void launch_one_thread_assign<T, D>(const DbscanArgs<T>&, const uint8_t*,            \
+                                                 const int32_t*, const int2*, int32_t,            \
+                                                 const int32_t*, int64_t*, hipStream_t);
 #define HPD_INST_DBSCAN_ALL_D(T) \
     HPD_INST_DBSCAN(T, 1)        \
     HPD_INST_DBSCAN(T, 2)        \

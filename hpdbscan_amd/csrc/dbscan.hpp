@@ -22,8 +22,8 @@ struct DbscanArgs {
     const int2* rcells = nullptr;    // [first, end) occupied cells of the same rows
     const int32_t* rcenter = nullptr; // index into ranges of each cell's own row
     int32_t m = 0;                   // occupied cells
-    T eps2{};                        // EPS2 converted to T
-    uint32_t min_pts = 0;
+    const JobParams<T>* jobs = nullptr;      // eps2, minPts, first row per job
+    const int32_t* job_of_cell = nullptr;    // job of every occupied cell
     // slot order: rows sorted by cell, then (when boxes are used) by sub-box
     const uint32_t* pos_of_slot = nullptr;  // sorted position per slot (null: identity)
     const uint32_t* inrow = nullptr;        // input row per slot
@@ -47,10 +47,28 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 // (box mode, bcluster != null) or compmin[find(parent, slot)]; then border
 // and noise labels.  Labels go to input order through a.inrow.
 // rootslot: ld entries.  stats[0] += noise rows, stats[1] += ambiguous
-// border rows.
+// border rows; when amb_list is given it receives (slot, cell) of each
+// ambiguous border row (capacity n).
 template <typename T, int D>
 void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                    int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                   int32_t* cmax, int64_t* labels, unsigned long long* stats, hipStream_t s);
+                   int32_t* cmax, int64_t* labels, unsigned long long* stats, int2* amb_list,
+                   hipStream_t s);
+
+// The reference's one-thread border choice for the namb ambiguous rows:
+// mark the touched clusters (involved, indexed by cluster position, zeroed
+// by the caller) and initialise f; relax f until *changed stays 0; assign.
+template <typename T, int D>
+void launch_one_thread_mark(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                            const int2* amb, int32_t namb, uint8_t* involved, int32_t* f,
+                            hipStream_t s);
+template <typename T, int D>
+void launch_one_thread_relax(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                             const uint8_t* involved, int32_t* f, unsigned int* changed,
+                             hipStream_t s);
+template <typename T, int D>
+void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* rootslot,
+                              const int2* amb, int32_t namb, const int32_t* f, int64_t* labels,
+                              hipStream_t s);
 
 }  // namespace hpd

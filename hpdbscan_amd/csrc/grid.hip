@@ -18,6 +18,7 @@ namespace hpd {
 namespace {
 
 constexpr int kMMThreads = 256;
+constexpr int kMMRows = 8 * kMMThreads;  // rows per partial block
 
 // The reference starts max at numeric_limits<T>::min(): FLT_MIN / DBL_MIN for
 // floating types (the smallest positive normal), the lowest value for
@@ -41,24 +42,44 @@ __device__ __forceinline__ T ref_max(T m, T c) {
     return (m < c) ? c : m;
 }
 
+// first job whose partial-block range contains block b (bfirst ascending)
+__device__ __forceinline__ int32_t job_of_block(const int32_t* __restrict__ bfirst, int32_t njobs,
+                                                int32_t b) {
+    int32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (bfirst[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One block reduces up to kMMRows rows of one job; bfirst[j] is the first
+// block of job j.
 template <typename T, int D>
-__global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restrict__ pts, int64_t n,
+__global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restrict__ pts,
+                                                               const JobParams<T>* __restrict__ jobs,
+                                                               int32_t njobs,
+                                                               const int32_t* __restrict__ bfirst,
                                                                T* __restrict__ part,
                                                                int* __restrict__ nonfinite) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x);
+    const int32_t j = job_of_block(bfirst, njobs, b);
+    const int64_t r0 = jobs[j].row_begin + static_cast<int64_t>(b - bfirst[j]) * kMMRows;
+    const int64_t r1 = min(r0 + kMMRows, jobs[j].row_end);
     T mn[D], mx[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        mn[j] = min_init<T>();
-        mx[j] = max_init<T>();
+    for (int k = 0; k < D; ++k) {
+        mn[k] = min_init<T>();
+        mx[k] = max_init<T>();
     }
     bool bad = false;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kMMThreads;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kMMThreads + threadIdx.x; i < n; i += stride) {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += kMMThreads) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            const T c = pts[i * D + j];
-            mn[j] = ref_min(mn[j], c);
-            mx[j] = ref_max(mx[j], c);
+        for (int k = 0; k < D; ++k) {
+            const T c = pts[i * D + k];
+            mn[k] = ref_min(mn[k], c);
+            mx[k] = ref_max(mx[k], c);
             if constexpr (std::is_floating_point<T>::value) bad |= !isfinite(c);
         }
     }
@@ -68,69 +89,72 @@ __global__ __launch_bounds__(kMMThreads) void k_minmax_partial(const T* __restri
     __shared__ T smn[D][kMMThreads];
     __shared__ T smx[D][kMMThreads];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        smn[j][threadIdx.x] = mn[j];
-        smx[j][threadIdx.x] = mx[j];
+    for (int k = 0; k < D; ++k) {
+        smn[k][threadIdx.x] = mn[k];
+        smx[k][threadIdx.x] = mx[k];
     }
     __syncthreads();
     for (int o = kMMThreads / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
 #pragma unroll
-            for (int j = 0; j < D; ++j) {
-                smn[j][threadIdx.x] = ref_min(smn[j][threadIdx.x], smn[j][threadIdx.x + o]);
-                smx[j][threadIdx.x] = ref_max(smx[j][threadIdx.x], smx[j][threadIdx.x + o]);
+            for (int k = 0; k < D; ++k) {
+                smn[k][threadIdx.x] = ref_min(smn[k][threadIdx.x], smn[k][threadIdx.x + o]);
+                smx[k][threadIdx.x] = ref_max(smx[k][threadIdx.x], smx[k][threadIdx.x + o]);
             }
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            part[static_cast<int64_t>(blockIdx.x) * 2 * D + j] = smn[j][0];
-            part[static_cast<int64_t>(blockIdx.x) * 2 * D + D + j] = smx[j][0];
+        for (int k = 0; k < D; ++k) {
+            part[static_cast<int64_t>(b) * 2 * D + k] = smn[k][0];
+            part[static_cast<int64_t>(b) * 2 * D + D + k] = smx[k][0];
         }
     }
 }
 
+// One block per job: min/max over the job's partial blocks.
 template <typename T, int D>
-__global__ __launch_bounds__(kMMThreads) void k_minmax_final(const T* __restrict__ part, int nblocks,
+__global__ __launch_bounds__(kMMThreads) void k_minmax_final(const T* __restrict__ part,
+                                                             const int32_t* __restrict__ bfirst,
                                                              T* __restrict__ out) {
+    const int32_t j = static_cast<int32_t>(blockIdx.x);
     __shared__ T smn[D][kMMThreads];
     __shared__ T smx[D][kMMThreads];
     T mn[D], mx[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        mn[j] = min_init<T>();
-        mx[j] = max_init<T>();
+    for (int k = 0; k < D; ++k) {
+        mn[k] = min_init<T>();
+        mx[k] = max_init<T>();
     }
-    for (int b = threadIdx.x; b < nblocks; b += kMMThreads) {
+    for (int32_t b = bfirst[j] + threadIdx.x; b < bfirst[j + 1]; b += kMMThreads) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            mn[j] = ref_min(mn[j], part[static_cast<int64_t>(b) * 2 * D + j]);
-            mx[j] = ref_max(mx[j], part[static_cast<int64_t>(b) * 2 * D + D + j]);
+        for (int k = 0; k < D; ++k) {
+            mn[k] = ref_min(mn[k], part[static_cast<int64_t>(b) * 2 * D + k]);
+            mx[k] = ref_max(mx[k], part[static_cast<int64_t>(b) * 2 * D + D + k]);
         }
     }
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        smn[j][threadIdx.x] = mn[j];
-        smx[j][threadIdx.x] = mx[j];
+    for (int k = 0; k < D; ++k) {
+        smn[k][threadIdx.x] = mn[k];
+        smx[k][threadIdx.x] = mx[k];
     }
     __syncthreads();
     for (int o = kMMThreads / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
 #pragma unroll
-            for (int j = 0; j < D; ++j) {
-                smn[j][threadIdx.x] = ref_min(smn[j][threadIdx.x], smn[j][threadIdx.x + o]);
-                smx[j][threadIdx.x] = ref_max(smx[j][threadIdx.x], smx[j][threadIdx.x + o]);
+            for (int k = 0; k < D; ++k) {
+                smn[k][threadIdx.x] = ref_min(smn[k][threadIdx.x], smn[k][threadIdx.x + o]);
+                smx[k][threadIdx.x] = ref_max(smx[k][threadIdx.x], smx[k][threadIdx.x + o]);
             }
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            out[j] = smn[j][0];
-            out[D + j] = smx[j][0];
+        for (int k = 0; k < D; ++k) {
+            out[static_cast<int64_t>(j) * 2 * D + k] = smn[k][0];
+            out[static_cast<int64_t>(j) * 2 * D + D + k] = smx[k][0];
         }
     }
 }
@@ -182,29 +206,30 @@ HPD_INT_CELL_INDEX(uint64_t)
 
 template <typename T, int D, typename K>
 __global__ __launch_bounds__(256) void k_cell_ids(const T* __restrict__ pts, int64_t n,
-                                                  const T* __restrict__ minv, CellParams cp,
-                                                  float rcp_f, double rcp_d, K* __restrict__ keys,
+                                                  const JobParams<T>* __restrict__ jobs,
+                                                  int32_t njobs, K* __restrict__ keys,
                                                   uint32_t* __restrict__ vals) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
+    const JobParams<T>& jp = jobs[job_of_row(jobs, njobs, i)];
     T row[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) row[j] = pts[i * D + j];
     uint64_t cell = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const int j = cp.swapped[k];
+        const int j = jp.cp.swapped[k];
         T xj = row[0];
-        T mj = minv[0];
+        T mj = jp.minv[0];
 #pragma unroll
         for (int t = 1; t < D; ++t)
             if (t == j) {
                 xj = row[t];
-                mj = minv[t];
+                mj = jp.minv[t];
             }
-        cell += cell_index<T>(xj, mj, rcp_f, rcp_d) * cp.stride[k];
+        cell += cell_index<T>(xj, mj, jp.rcp_f, jp.rcp_d) * jp.cp.stride[k];
     }
-    keys[i] = static_cast<K>(cell);
+    keys[i] = static_cast<K>(jp.key_base + cell);
     vals[i] = static_cast<uint32_t>(i);
 }
 
@@ -223,26 +248,24 @@ inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n +
 
 }  // namespace
 
-int minmax_partial_blocks(int64_t n) {
-    const int64_t b = (n + kMMThreads * 8 - 1) / (kMMThreads * 8);
-    return static_cast<int>(b < 1 ? 1 : (b > 1024 ? 1024 : b));
-}
+int minmax_rows_per_block() { return kMMRows; }
 
 template <typename T, int D>
-void launch_minmax(const T* pts, int64_t n, T* part, T* out, int* nonfinite, hipStream_t s) {
-    const int nb = minmax_partial_blocks(n);
-    hipLaunchKernelGGL((k_minmax_partial<T, D>), dim3(nb), dim3(kMMThreads), 0, s, pts, n, part,
-                       nonfinite);
-    hipLaunchKernelGGL((k_minmax_final<T, D>), dim3(1), dim3(kMMThreads), 0, s, part, nb, out);
+void launch_minmax(const T* pts, const JobParams<T>* jobs, int32_t njobs, const int32_t* bfirst,
+                   int32_t nblocks, T* part, T* out, int* nonfinite, hipStream_t s) {
+    if (nblocks > 0)
+        hipLaunchKernelGGL((k_minmax_partial<T, D>), dim3(nblocks), dim3(kMMThreads), 0, s, pts, jobs,
+                           njobs, bfirst, part, nonfinite);
+    hipLaunchKernelGGL((k_minmax_final<T, D>), dim3(njobs), dim3(kMMThreads), 0, s, part, bfirst, out);
     HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D, typename K>
-void launch_cell_ids(const T* pts, int64_t n, const T* minv, const CellParams& cp, float rcp_f,
-                     double rcp_d, K* keys, uint32_t* vals, hipStream_t s) {
+void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, K* keys,
+                     uint32_t* vals, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL((k_cell_ids<T, D, K>), dim3(blocks_for(n, 256)), dim3(256), 0, s, pts, n,
-                       minv, cp, rcp_f, rcp_d, keys, vals);
+                       jobs, njobs, keys, vals);
     HPD_HIP(hipGetLastError());
 }
 
@@ -256,13 +279,12 @@ void launch_gather_soa(const T* pts, int64_t n, const uint32_t* order, T* soa, i
 }
 
 #define HPD_INST_GRID(T, D)                                                                     \
-    template void launch_minmax<T, D>(const T*, int64_t, T*, T*, int*, hipStream_t);                         \
-    template void launch_cell_ids<T, D, uint32_t>(const T*, int64_t, const T*, const CellParams&, \
-                                                  float, double, uint32_t*, uint32_t*,          \
-                                                  hipStream_t);                                 \
-    template void launch_cell_ids<T, D, uint64_t>(const T*, int64_t, const T*, const CellParams&, \
-                                                  float, double, uint64_t*, uint32_t*,          \
-                                                  hipStream_t);                                 \
+    template void launch_minmax<T, D>(const T*, const JobParams<T>*, int32_t, const int32_t*,    \
+                                      int32_t, T*, T*, int*, hipStream_t);                       \
+    template void launch_cell_ids<T, D, uint32_t>(const T*, int64_t, const JobParams<T>*,        \
+                                                  int32_t, uint32_t*, uint32_t*, hipStream_t);   \
+    template void launch_cell_ids<T, D, uint64_t>(const T*, int64_t, const JobParams<T>*,        \
+                                                  int32_t, uint64_t*, uint32_t*, hipStream_t);   \
     template void launch_gather_soa<T, D>(const T*, int64_t, const uint32_t*, T*, int64_t,      \
                                           hipStream_t);
 

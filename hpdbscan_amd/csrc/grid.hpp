@@ -23,17 +23,22 @@
 
 namespace hpd {
 
-int minmax_partial_blocks(int64_t n);
+// rows reduced by one block of the min/max kernel
+int minmax_rows_per_block();
 
-// minmax: part holds 2*D*minmax_partial_blocks(n) values; out receives
-// min[0..D) then max[0..D); *nonfinite (zeroed by the caller) becomes 1 if a
-// coordinate is NaN or infinite.
+// Per job j: out[j*2D .. j*2D+D) = min, out[j*2D+D .. (j+1)*2D) = max over
+// the job's rows.  bfirst[j] = first partial block of job j (host-made,
+// ceil(rows / minmax_rows_per_block()) blocks per job, bfirst[njobs] =
+// nblocks); part holds 2*D*nblocks values; *nonfinite (zeroed by the caller)
+// becomes 1 if a coordinate is NaN or infinite.
 template <typename T, int D>
-void launch_minmax(const T* pts, int64_t n, T* part, T* out, int* nonfinite, hipStream_t s);
+void launch_minmax(const T* pts, const JobParams<T>* jobs, int32_t njobs, const int32_t* bfirst,
+                   int32_t nblocks, T* part, T* out, int* nonfinite, hipStream_t s);
 
+// key = job key_base + linearised cell id (spatial_index.h:144-165)
 template <typename T, int D, typename K>
-void launch_cell_ids(const T* pts, int64_t n, const T* minv, const CellParams& cp, float rcp_f,
-                     double rcp_d, K* keys, uint32_t* vals, hipStream_t s);
+void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, K* keys,
+                     uint32_t* vals, hipStream_t s);
 
 // soa[j * ld + i] = pts[order[i] * D + j]
 template <typename T, int D>

@@ -16,6 +16,7 @@
 // table.  A cell's ranges are stored in ascending row order; the index of
 // the row holding the cell itself is kept separately.
 #include "common.hpp"
+#include "grid.hpp"
 #include "index.hpp"
 #include "internal.hpp"
 
@@ -108,20 +109,42 @@ __device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp,
     }
 }
 
-template <typename K>
+// Job of an occupied cell: the last j with key_base <= key.  Jobs without
+// rows get the key_base of the job after them, so that job is never one of
+// them.
+template <typename T>
+__device__ __forceinline__ int32_t job_of_key(const JobParams<T>* __restrict__ jobs, int32_t njobs,
+                                              uint64_t key) {
+    int32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].key_base <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename K, typename T>
 __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict__ ucell,
                                                              const int32_t* __restrict__ ustart,
-                                                             int32_t m, CellParams cp,
+                                                             int32_t m,
+                                                             const JobParams<T>* __restrict__ jobs,
+                                                             int32_t njobs,
+                                                             int32_t* __restrict__ job_of_cell,
                                                              uint32_t* __restrict__ nranges,
                                                              unsigned long long* __restrict__ sumcand) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
     if (u < m) {
+        const uint64_t key = static_cast<uint64_t>(ucell[u]);
+        const int32_t j = job_of_key(jobs, njobs, key);
+        job_of_cell[u] = j;
+        const uint64_t base = jobs[j].key_base;
         uint32_t cnt = 0;
-        for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi, bool) {
-            const int32_t a = lower_bound_cells(ucell, m, lo);
-            const int32_t b = lower_bound_cells(ucell, m, hi + 1);
+        for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool) {
+            const int32_t a = lower_bound_cells(ucell, m, base + lo);
+            const int32_t b = lower_bound_cells(ucell, m, base + hi + 1);
             if (b > a) {
                 ++cnt;
                 cand += static_cast<uint64_t>(ustart[b] - ustart[a]);
@@ -137,27 +160,32 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
     if (__lane_id() == 0 && v) atomicAdd(sumcand, v);
 }
 
-template <typename K>
+template <typename K, typename T>
 __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict__ ucell,
                                                             const int32_t* __restrict__ ustart,
-                                                            int32_t m, CellParams cp,
+                                                            int32_t m,
+                                                            const JobParams<T>* __restrict__ jobs,
+                                                            const int32_t* __restrict__ job_of_cell,
                                                             const uint32_t* __restrict__ roff,
                                                             int2* __restrict__ ranges,
                                                             int2* __restrict__ rcells,
                                                             int32_t* __restrict__ rcenter) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
+    const int32_t j = job_of_cell[u];
+    const uint64_t base = jobs[j].key_base;
     uint32_t w = roff[u];
     int32_t center = static_cast<int32_t>(w);
-    for_each_row(static_cast<uint64_t>(ucell[u]), cp, [&](uint64_t lo, uint64_t hi, bool is_center) {
-        const int32_t a = lower_bound_cells(ucell, m, lo);
-        const int32_t b = lower_bound_cells(ucell, m, hi + 1);
-        if (b > a) {
-            if (is_center) center = static_cast<int32_t>(w);
-            rcells[w] = make_int2(a, b);
-            ranges[w++] = make_int2(ustart[a], ustart[b]);
-        }
-    });
+    for_each_row(static_cast<uint64_t>(ucell[u]) - base, jobs[j].cp,
+                 [&](uint64_t lo, uint64_t hi, bool is_center) {
+                     const int32_t a = lower_bound_cells(ucell, m, base + lo);
+                     const int32_t b = lower_bound_cells(ucell, m, base + hi + 1);
+                     if (b > a) {
+                         if (is_center) center = static_cast<int32_t>(w);
+                         rcells[w] = make_int2(a, b);
+                         ranges[w++] = make_int2(ustart[a], ustart[b]);
+                     }
+                 });
     rcenter[u] = center;
 }
 
@@ -192,20 +220,22 @@ void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp,
     HPD_HIP(hipGetLastError());
 }
 
-template <typename K>
-void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+template <typename K, typename T>
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
+                           const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, unsigned long long* sumcand, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbor_rows_count<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, ucell,
-                       ustart, m, cp, nranges, sumcand);
+    hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
+                       ucell, ustart, m, jobs, njobs, job_of_cell, nranges, sumcand);
     HPD_HIP(hipGetLastError());
 }
 
-template <typename K>
-void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+template <typename K, typename T>
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m,
+                          const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbor_rows_fill<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, ucell,
-                       ustart, m, cp, roff, ranges, rcells, rcenter);
+    hipLaunchKernelGGL((k_neighbor_rows_fill<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
+                       ucell, ustart, m, jobs, job_of_cell, roff, ranges, rcells, rcenter);
     HPD_HIP(hipGetLastError());
 }
 
@@ -222,14 +252,29 @@ void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, i
     HPD_HIP(hipGetLastError());
 }
 
-#define HPD_INST_INDEX(K)                                                                        \
-    template void launch_cells<K>(const K*, int64_t, uint32_t*, uint32_t*, K*, int32_t*,         \
-                                  uint32_t*, hipStream_t);                                       \
-    template void launch_neighbor_count<K>(const K*, const int32_t*, int32_t, const CellParams&, \
-                                           uint32_t*, unsigned long long*, hipStream_t);         \
-    template void launch_neighbor_fill<K>(const K*, const int32_t*, int32_t, const CellParams&,  \
-                                          const uint32_t*, int2*, int2*, int32_t*, hipStream_t);
+#define HPD_INST_INDEX(K)                                                                     \
+    template void launch_cells<K>(const K*, int64_t, uint32_t*, uint32_t*, K*, int32_t*,      \
+                                  uint32_t*, hipStream_t);
 HPD_INST_INDEX(uint32_t)
 HPD_INST_INDEX(uint64_t)
+
+#define HPD_INST_ROWS(T)                                                                        \
+    template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
+                                                     const JobParams<T>*, int32_t, int32_t*,    \
+                                                     uint32_t*, unsigned long long*,            \
+                                                     hipStream_t);                              \
+    template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
+                                                     const JobParams<T>*, int32_t, int32_t*,    \
+                                                     uint32_t*, unsigned long long*,            \
+                                                     hipStream_t);                              \
+    template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
+                                                    const JobParams<T>*, const int32_t*,        \
+                                                    const uint32_t*, int2*, int2*, int32_t*,    \
+                                                    hipStream_t);                               \
+    template void launch_neighbor_fill<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,   \
+                                                    const JobParams<T>*, const int32_t*,        \
+                                                    const uint32_t*, int2*, int2*, int32_t*,    \
+                                                    hipStream_t);
+HPD_FOR_EACH_DTYPE(HPD_INST_ROWS)
 
 }  // namespace hpd

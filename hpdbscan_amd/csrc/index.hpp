@@ -15,17 +15,19 @@ template <typename K>
 void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,
                   int32_t* ustart, uint32_t* ncells_dev, hipStream_t s);
 
-// nranges[u] = number of non-empty neighbour rows of cell u; adds
-// Σ_u count(u) * |candidates(u)| to *sumcand.
-template <typename K>
-void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+// job_of_cell[u] = job of cell u; nranges[u] = number of non-empty
+// neighbour rows of cell u; adds Σ_u count(u) * |candidates(u)| to *sumcand.
+template <typename K, typename T>
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
+                           const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, unsigned long long* sumcand, hipStream_t s);
 
 // ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row,
 // ascending; rcells[..] = [first, end) occupied-cell index of the same row;
 // rcenter[u] = index (into ranges) of the row holding cell u itself
-template <typename K>
-void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m, const CellParams& cp,
+template <typename K, typename T>
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m,
+                          const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
                           hipStream_t s);
 

@@ -80,12 +80,44 @@ struct Geometry {
     int key_bits = 0;                // bits of the largest cell id
 };
 
-// Device-side parameter blocks (passed by value to kernels).
+// Device-side parameter blocks.
 struct CellParams {
     int d;
     int swapped[kMaxDims];
     uint64_t stride[kMaxDims];
     uint64_t ncell[kMaxDims];
 };
+
+// Everything a kernel needs to know about one clustering job (one point
+// cloud with its own eps and minPts).  A plain call is a batch of one job.
+// Jobs are laid out back to back: rows [row_begin, row_end) of the
+// concatenated input, cell keys [key_base, key_base + total cells).  After
+// the sort by key, a job's sorted positions are again [row_begin, row_end).
+template <typename T>
+struct JobParams {
+    CellParams cp;
+    uint64_t key_base = 0;
+    int64_t row_begin = 0;
+    int64_t row_end = 0;
+    T minv[kMaxDims] = {};  // per-dimension minimum: the grid origin
+    T eps2{};               // EPS2 converted to T
+    float rcp_f = 0.0f;     // 1/eps in float (float and integer data)
+    double rcp_d = 0.0;     // 1/eps in double (double data)
+    double gap2_max = 0.0;  // box pairs farther apart than this cannot connect
+    uint32_t min_pts = 0;
+};
+
+// job index of a row (rows sorted by job): last j with row_begin <= row
+template <typename T>
+__device__ __forceinline__ int32_t job_of_row(const JobParams<T>* __restrict__ jobs, int32_t njobs,
+                                              int64_t row) {
+    int32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row_begin <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 }  // namespace hpd

@@ -93,12 +93,14 @@ def cloud(n: int, d: int, k: int, rng: np.random.Generator, sigma: float = 0.2,
           box: float = 10.0, noise: float = 0.1) -> np.ndarray:
     """K Gaussian blobs (centres uniform in [0, box]^d, isotropic sigma) plus a
     `noise` fraction of uniform points; float32, shuffled."""
-    centres = rng.uniform(0.0, box, size=(k, d))
+    centres = rng.uniform(0.0, box, size=(k, d)).astype(np.float32)
     n_noise = int(round(n * noise))
-    which = rng.integers(0, k, size=n - n_noise)
-    pts = centres[which] + rng.normal(0.0, sigma, size=(n - n_noise, d))
-    x = np.concatenate([pts, rng.uniform(0.0, box, size=(n_noise, d))])
-    return _finish(x, rng)
+    x = np.empty((n, d), dtype=np.float32)
+    blk = x[: n - n_noise]
+    np.multiply(rng.standard_normal((n - n_noise, d), dtype=np.float32), np.float32(sigma), out=blk)
+    blk += centres[rng.integers(0, k, size=n - n_noise)]
+    np.multiply(rng.random((n_noise, d), dtype=np.float32), np.float32(box), out=x[n - n_noise:])
+    return x[rng.permutation(n)]
 
 
 def make(name: str, n: int | None = None) -> tuple[np.ndarray, float, int]:
@@ -107,3 +109,52 @@ def make(name: str, n: int | None = None) -> tuple[np.ndarray, float, int]:
     fn = {"blobs_100k": blobs_100k, "blobs_10m": blobs_10m, "skewed_2d": skewed_2d}[name]
     x = fn(n=n or w["n"], seed=w["seed"])
     return x, w["eps"], w["min_points"]
+
+
+def batch_clouds(n_jobs: int = 4096, seed: int = 4, n_min: int = 1_000, n_max: int = 200_000):
+    """The batched workload: n_jobs clouds in 3-D, sizes log-uniform in
+    [n_min, n_max], each 5-50 Gaussian blobs (centres uniform in [0,10]^3,
+    sigma 0.2, equal weights) + 10 % uniform noise in [0,10]^3; eps uniform
+    in [0.2, 0.5], minPoints in {4, 8, 16}.  Within a cloud the blob rows
+    come first, then the noise rows.  Returns (rows, offsets, eps,
+    min_points)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(n_min), np.log(n_max), size=n_jobs))).astype(np.int64)
+    offsets = np.zeros(n_jobs + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(sizes)
+    rows = np.empty((int(offsets[-1]), 3), dtype=np.float32)
+    for j in range(n_jobs):
+        n = int(sizes[j])
+        nn = int(round(n * 0.1))
+        nb = n - nn
+        k = int(rng.integers(5, 51))
+        centres = rng.uniform(0.0, 10.0, size=(k, 3)).astype(np.float32)
+        o = int(offsets[j])
+        blk = rows[o:o + nb]
+        np.multiply(rng.standard_normal((nb, 3), dtype=np.float32), np.float32(0.2), out=blk)
+        blk += centres[rng.integers(0, k, size=nb)]
+        np.multiply(rng.random((nn, 3), dtype=np.float32), np.float32(10.0), out=rows[o + nb:o + n])
+    eps = rng.uniform(0.2, 0.5, size=n_jobs).astype(np.float32)
+    mps = rng.choice(np.array([4, 8, 16], dtype=np.uint64), size=n_jobs)
+    return rows, offsets, eps, mps
+
+
+def serving_stream(n_requests: int = 400, rate: float = 200.0, seed: int = 5,
+                   n_min: int = 10_000, n_max: int = 5_000_000, pool_per_d: int = 6):
+    """The serving workload: Poisson arrivals at `rate` per second; sizes
+    log-uniform in [n_min, n_max]; d uniform in {2, 3, 4}; eps uniform in
+    [0.2, 0.5]; minPoints uniform in 4..32.  Each request's rows are the
+    first n rows of one of `pool_per_d` shuffled blob mixtures (5-50 blobs,
+    sigma 0.2, 10 % noise in [0,10]^d) of its dimension.
+    Returns a list of (arrival_s, rows, eps, min_points)."""
+    rng = np.random.default_rng(seed)
+    pools = {d: [cloud(n_max, d, int(rng.integers(5, 51)), rng) for _ in range(pool_per_d)]
+             for d in (2, 3, 4)}
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    out = []
+    for i in range(n_requests):
+        d = int(rng.choice([2, 3, 4]))
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        base = pools[d][int(rng.integers(0, pool_per_d))]
+        out.append((float(t[i]), base[:n], float(rng.uniform(0.2, 0.5)), int(rng.integers(4, 33))))
+    return out

@@ -31,7 +31,8 @@ typedef enum hpd_status {
     HPD_EINVAL = 1,      /* maps to std::invalid_argument */
     HPD_ENOMEM = 2,      /* device or host allocation failed */
     HPD_EDEVICE = 3,     /* HIP runtime error or no usable device */
-    HPD_EUNSUPPORTED = 4 /* valid input this build does not handle */
+    HPD_EUNSUPPORTED = 4, /* valid input this build does not handle */
+    HPD_ETIMEOUT = 5      /* hpd_server_wait: the request is not finished yet */
 } hpd_status;
 
 /* Element types of the input rows; the reference instantiates the same ten
@@ -131,6 +132,83 @@ int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_
  * and labels_out is complete when the call returns. */
 int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_t d,
                        float eps, uint64_t min_pts, int64_t* labels_out, void* stream);
+
+/*
+ * Batched clustering: njobs independent point clouds of the same dtype and
+ * dimension, concatenated row-wise; job j is rows [offsets[j], offsets[j+1])
+ * (offsets has njobs + 1 entries, offsets[0] = 0) with its own eps[j] and
+ * min_pts[j].  labels_out[offsets[j] ..] receives exactly the labels
+ * hpd_cluster would return for job j alone (cluster ids count from the
+ * job's own sorted positions).  All jobs run through one launch of every
+ * kernel.  No counterpart in the reference, which clusters one dataset per
+ * call (hpdbscan.h:475-486); offsets, eps and min_pts are host arrays.
+ */
+int hpd_cluster_batch(hpd_context* ctx, int dtype, const void* data, const int64_t* offsets,
+                      int32_t njobs, int32_t d, const float* eps, const uint64_t* min_pts,
+                      int64_t* labels_out);
+
+/* Same with `data` and `labels_out` in device memory; ordered on `stream`
+ * (NULL: the context's stream), complete on return. */
+int hpd_cluster_batch_device(hpd_context* ctx, int dtype, const void* data,
+                             const int64_t* offsets, int32_t njobs, int32_t d, const float* eps,
+                             const uint64_t* min_pts, int64_t* labels_out, void* stream);
+
+/*
+ * Continuous-batching server (no counterpart in the reference).
+ *
+ * hpd_server_submit copies one point cloud and returns a ticket at once.  A
+ * scheduler groups waiting requests of the same dtype and dimension into a
+ * batch whenever one of `lanes` workers is idle; each worker owns a context
+ * (HIP stream + workspace) and runs its batch with hpd_cluster_batch, so the
+ * batches of different lanes overlap on the GPU.  Admission charges every
+ * request rows * (bytes_per_row_estimate + row bytes) against
+ * hbm_budget_bytes; a batch waits until it fits beside the running ones, a
+ * request larger than the whole budget (or than max_batch_rows) is refused
+ * with HPD_ENOMEM at submit.  Results are identical to hpd_cluster on the
+ * same request alone.  Destroying the server finishes running batches and
+ * drops queued requests; do not wait on tickets while destroying.
+ */
+typedef struct hpd_server hpd_server;
+
+typedef struct hpd_server_config {
+    int device;                     /* HIP device */
+    int lanes;                      /* concurrent batches (contexts / streams) */
+    size_t hbm_budget_bytes;        /* device memory all running batches may use */
+    size_t bytes_per_row_estimate;  /* workspace charged per row (plus the row itself) */
+    int32_t max_batch_jobs;         /* requests per batch */
+    int64_t max_batch_rows;         /* rows per batch */
+} hpd_server_config;
+
+typedef struct hpd_request_info {
+    int64_t rows;
+    int32_t batch_jobs;  /* requests in the batch it ran in */
+    double queue_ms;     /* submit -> batch start */
+    double latency_ms;   /* submit -> result ready */
+} hpd_request_info;
+
+typedef struct hpd_server_stats {
+    uint64_t completed, failed, batches;
+    double mean_batch_jobs;
+    int64_t rows;
+    double p50_ms, p99_ms, max_ms;  /* submit -> result latency of completed requests */
+    double wall_s;                  /* first submit -> last completion */
+    double jobs_per_s, mrows_per_s;
+    uint64_t queued;
+    size_t charged_bytes;
+} hpd_server_stats;
+
+void hpd_server_default_config(hpd_server_config* cfg);
+int hpd_server_create(const hpd_server_config* cfg, hpd_server** out);
+int hpd_server_submit(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
+                      float eps, uint64_t min_pts, uint64_t* ticket);
+/* Wait for a ticket (timeout_ms < 0: forever) and copy its n labels.
+ * HPD_ETIMEOUT leaves the ticket valid; any other return consumes it. */
+int hpd_server_wait(hpd_server* srv, uint64_t ticket, int64_t* labels_out, int64_t n,
+                    double timeout_ms, hpd_request_info* info);
+int hpd_server_get_stats(hpd_server* srv, hpd_server_stats* out);
+int hpd_server_destroy(hpd_server* srv);
+/* Thread-local message of the last failing hpd_server_* call. */
+const char* hpd_server_last_error(void);
 
 /* Device-memory helpers for callers of hpd_cluster_device that have no HIP
  * runtime of their own (no counterpart in the reference).  Allocations are

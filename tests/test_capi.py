@@ -126,3 +126,16 @@ def test_cxx_header_compiles():
         assert r.returncode == 0, r.stderr
         r = subprocess.run([exe], capture_output=True, text=True)
         assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+
+
+def test_batch_argument_errors():
+    from hpdbscan_amd import _lib
+    lib = _lib.load()
+    x = np.zeros((4, 2), np.float32)
+    off = np.array([0, 2, 4], np.int64)
+    eps = np.array([0.5, 0.0], np.float32)
+    mp = np.array([2, 2], np.uint64)
+    out = np.zeros(4, np.int64)
+    rc = lib.hpd_cluster_batch(None, 0, x.ctypes.data, off.ctypes.data, 2, 2, eps.ctypes.data,
+                               mp.ctypes.data, out.ctypes.data)
+    assert rc == _lib.HPD_EINVAL

@@ -4,7 +4,8 @@ reference, on an MI355X.
 Bar: core/noise flags, core cluster ids and one-cluster border labels are
 identical to the reference's one-thread labels; a border row next to cores
 of two clusters carries the id of one of those clusters (the reference's own
-multi-thread behaviour), checked by oracle.check.
+multi-thread behaviour), checked by oracle.check.  With the one-thread border
+rule every label equals the reference's one-thread label.
 """
 from __future__ import annotations
 
@@ -188,3 +189,162 @@ def test_skewed_2d_sample(ctx):
     say("skewed 2-D sample n=200000 eps=0.01 minPts=32")
     got = ctx.cluster(x, 0.01, 32)
     assert_parity(x, 0.01, 32, got, oracle.cluster(x, 0.01, 32))
+
+
+def test_batch_matches_single_jobs(ctx):
+    """One batched call returns, for every job, the labels of clustering the
+    job alone (and the reference's labels)."""
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(17)
+    clouds, eps, mps = [], [], []
+    for j in range(24):
+        n = int(np.exp(rng.uniform(np.log(50), np.log(6000))))
+        if j == 5:
+            n = 0
+        if j == 9:
+            n = 1
+        clouds.append(blobs(n, 3, int(rng.integers(2, 10)), 0.2, 6.0, 1000 + j, np.float32)
+                      if n > 1 else rng.uniform(0, 5, (n, 3)).astype(np.float32))
+        eps.append(float(rng.uniform(0.2, 0.5)))
+        mps.append(int(rng.choice([4, 8, 16])))
+    x, off = _lib.concat_jobs(clouds)
+    say(f"batch of {len(clouds)} jobs, {len(x)} rows")
+    got = ctx.cluster_batch(x, off, eps, mps)
+    for j, c in enumerate(clouds):
+        g = got[off[j]:off[j + 1]]
+        if len(c) == 0:
+            continue
+        np.testing.assert_array_equal(g, ctx.cluster(c, eps[j], mps[j]))
+        assert_parity(c, eps[j], mps[j], g, oracle.cluster(c, eps[j], mps[j]))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.int32])
+def test_batch_other_dtypes(ctx, dtype):
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(5)
+    clouds = []
+    for j in range(6):
+        if np.dtype(dtype).kind == "f":
+            clouds.append(blobs(int(rng.integers(500, 3000)), 2, 5, 0.3, 5.0, j, dtype))
+        else:
+            clouds.append(rng.integers(0, 60, size=(int(rng.integers(500, 3000)), 2)).astype(dtype))
+    eps = [0.4, 0.3, 2.5, 0.35, 3.0, 0.5]
+    x, off = _lib.concat_jobs(clouds)
+    say(f"batch dtype={np.dtype(dtype).name}")
+    got = ctx.cluster_batch(x, off, eps, 5)
+    for j, c in enumerate(clouds):
+        assert_parity(c, eps[j], 5, got[off[j]:off[j + 1]], oracle.cluster(c, eps[j], 5))
+
+
+# ---------------------------------------------------------------------------
+# HPD_BORDER_ONE_THREAD: every label, ambiguous borders included, equals the
+# reference's at one thread.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def ctx1():
+    from hpdbscan_amd import _lib
+    c = _lib.Context(0)
+    c.set_border_rule(_lib.BORDER_ONE_THREAD)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_one_thread_rule_golden(ctx1, name):
+    z = np.load(os.path.join(GOLDEN, name))
+    x, eps, mp = z["data"], float(z["eps"]), int(z["min_pts"])
+    say(f"one-thread rule fixture {name}")
+    np.testing.assert_array_equal(ctx1.cluster(x, eps, mp), z["labels"])
+
+
+# (n, d, clusters, sigma, box, seed, dtype, eps, minPts): overlapping blobs
+# with 30 % uniform noise, chosen so that borders next to two clusters occur
+# and the smallest-cluster choice differs from the one-thread choice
+ONE_THREAD_CASES = [(1500, 2, 40, 0.35, 6.0, s, np.float32, 0.25, 6) for s in range(6)] + [
+    (20000, 2, 60, 0.3, 10.0, 1, np.float32, 0.12, 8),
+    (20000, 3, 60, 0.3, 8.0, 1, np.float64, 0.2, 10),
+    (30000, 4, 80, 0.3, 7.0, 1, np.float32, 0.35, 12),
+    (8000, 1, 30, 0.4, 40.0, 2, np.float32, 0.02, 4),
+    (6000, 6, 40, 0.5, 5.0, 3, np.float64, 0.9, 8),
+]
+
+
+@pytest.mark.parametrize("case", ONE_THREAD_CASES, ids=lambda c: f"n{c[0]}d{c[1]}s{c[5]}")
+def test_one_thread_rule_exact(ctx1, case):
+    n, d, k, sig, box, seed, dt, eps, mp = case
+    x = blobs(n, d, k, sig, box, seed, dt, noise=0.3)
+    say(f"one-thread rule n={n} d={d} seed={seed} dtype={np.dtype(dt).name}")
+    full = oracle.cluster(x, eps, mp, full=True)
+    got = ctx1.cluster(x, eps, mp)
+    np.testing.assert_array_equal(got, full["labels"])
+    assert ctx1.stats()["ambiguous_borders"] == full["stats"]["ambiguous_borders"]
+
+
+def test_one_thread_rule_discriminates(ctx, ctx1):
+    """The cases above hold borders where the two rules choose differently."""
+    differ = 0
+    for case in ONE_THREAD_CASES[:6]:
+        n, d, k, sig, box, seed, dt, eps, mp = case
+        x = blobs(n, d, k, sig, box, seed, dt, noise=0.3)
+        differ += int(np.count_nonzero(ctx.cluster(x, eps, mp) != ctx1.cluster(x, eps, mp)))
+    say(f"rows where the border rules differ: {differ}")
+    assert differ > 0
+
+
+def test_one_thread_rule_point_union_and_integers(ctx1):
+    rng = np.random.default_rng(3)
+    base = np.float32(6.0e6)
+    x = np.stack([base + rng.integers(0, 4000, 6000).astype(np.float32) * np.float32(0.5),
+                  (rng.integers(0, 3, 6000) * 0.1).astype(np.float32)], axis=1).astype(np.float32)
+    x = np.concatenate([x, np.zeros((1, 2), np.float32)])
+    say("one-thread rule, loose boxes")
+    np.testing.assert_array_equal(ctx1.cluster(x, 0.3, 3), oracle.cluster(x, 0.3, 3))
+    assert ctx1.stats()["box_union"] == 0
+    xi = (blobs(6000, 2, 30, 3.0, 100.0, 4, np.float64, noise=0.3)).astype(np.int32)
+    say("one-thread rule, int32")
+    np.testing.assert_array_equal(ctx1.cluster(xi, 2.5, 5), oracle.cluster(xi, 2.5, 5))
+
+
+def test_one_thread_rule_batch(ctx1):
+    from hpdbscan_amd import _lib
+    clouds = [blobs(1500, 2, 40, 0.35, 6.0, s, np.float32, noise=0.3) for s in range(4)]
+    x, off = _lib.concat_jobs(clouds)
+    say("one-thread rule, batch of 4")
+    got = ctx1.cluster_batch(x, off, 0.25, 6)
+    for j, c in enumerate(clouds):
+        np.testing.assert_array_equal(got[off[j]:off[j + 1]], oracle.cluster(c, 0.25, 6))
+
+
+def test_server_results_equal_alone_and_in_the_mix(ctx):
+    """Requests of mixed sizes and dimensions submitted at once come back
+    identical to clustering each request alone."""
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(23)
+    reqs = []
+    for i in range(30):
+        d = int(rng.choice([2, 3, 4]))
+        n = int(np.exp(rng.uniform(np.log(200), np.log(20000))))
+        x = blobs(n, d, int(rng.integers(5, 20)), 0.2, 10.0, 300 + i, np.float32)
+        reqs.append((x, float(rng.uniform(0.2, 0.5)), int(rng.integers(4, 33))))
+    say(f"server: {len(reqs)} requests")
+    with _lib.Server(lanes=3) as srv:
+        tickets = [srv.submit(x, e, m) for x, e, m in reqs]
+        outs = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["batches"] < len(reqs)  # requests were batched
+    for (x, e, m), (got, info) in zip(reqs, outs):
+        np.testing.assert_array_equal(got, ctx.cluster(x, e, m))
+
+
+def test_server_admission_rejects_oversize(ctx):
+    from hpdbscan_amd import _lib
+    say("server admission")
+    with _lib.Server(lanes=1, hbm_budget_bytes=1 << 20) as srv:
+        x = np.zeros((100000, 3), np.float32)
+        with pytest.raises(MemoryError):
+            srv.submit(x, 0.3, 4)
+        small = blobs(500, 3, 3, 0.2, 3.0, 1, np.float32)
+        t = srv.submit(small, 0.3, 4)
+        got, info = srv.wait(t)
+        np.testing.assert_array_equal(got, ctx.cluster(small, 0.3, 4))
