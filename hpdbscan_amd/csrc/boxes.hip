@@ -20,7 +20,8 @@
 // The union of the cores is therefore computed on boxes: two boxes with
 // cores are joined when some core pair across them is within eps (tested
 // with the reference's expression, stopping at the first hit), unless they
-// are already joined.  Box pairs whose bounding boxes are farther apart than
+// are already joined; a core row farther than eps from the other box's
+// bounding box takes no part in the test.  Box pairs whose bounding boxes are farther apart than
 // eps * (1 + 1e-5) cannot hold such a pair (the float result of the
 // expression is within a few ulps of the exact squared distance) and are
 // skipped.
@@ -30,6 +31,7 @@
 #include "common.hpp"
 #include "grid.hpp"
 #include "internal.hpp"
+#include "visit.hpp"
 
 namespace hpd {
 
@@ -139,27 +141,6 @@ __global__ __launch_bounds__(kThreads) void k_cell_boxes(const int32_t* __restri
     if (b == 0) cbox[m] = nbox;
 }
 
-// order-preserving integer images of floating-point values (for atomics)
-__device__ __forceinline__ uint32_t ord_key(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord_val(uint32_t k) {
-    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
-}
-__device__ __forceinline__ unsigned long long ord_key(double f) {
-    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(f));
-    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double ord_val(unsigned long long k) {
-    return __longlong_as_double(static_cast<long long>((k & 0x8000000000000000ull)
-                                                           ? (k & 0x7fffffffffffffffull)
-                                                           : ~k));
-}
-
-template <typename T>
-using OrdKey = std::conditional_t<sizeof(T) == 8, unsigned long long, uint32_t>;
-
 template <typename U>
 __device__ __forceinline__ U wave_min_u(U v) {
 #pragma unroll
@@ -246,70 +227,150 @@ __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restr
     if (!tight && size > 1) atomicAdd(nloose, 1u);
 }
 
-__device__ __forceinline__ int32_t find_root(int32_t* parent, int32_t x) { return uf_find(parent, x); }
-
-// Box-level union: one thread per box with cores.  Neighbour boxes come
-// from the cell's neighbour rows (cells [a, b) -> boxes [cbox[a], cbox[b])).
+// Box-level hook: every box with cores points at some box of a smaller
+// index holding a core within eps of one of its cores (found by the box's
+// chunks, each stopping at its first hit; any such box will do).  parent
+// stays <= the box index, so the hooks form a forest whose trees are
+// connected sets of boxes.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_box_union(
-    const T* __restrict__ soa, int64_t ld, int32_t nbox, const int32_t* __restrict__ bstart,
-    const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
-    const uint32_t* __restrict__ roff, const int2* __restrict__ rcells,
-    const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ bmincore,
-    const uint8_t* __restrict__ core, const int32_t* __restrict__ job_of_cell,
-    const JobParams<T>* __restrict__ jobs, int32_t* bparent) {
-    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (b >= nbox || bmincore[b] == INT_MAX) return;
-    const JobParams<T>& jp = jobs[job_of_cell[bcell[b]]];
+__global__ __launch_bounds__(kThreads) void k_box_hook(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
+    const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
+    const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
+    int32_t* __restrict__ bparent) {
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
+    if (wid >= nitems) return;
+    const int2 it = bitems[wid];
+    const int32_t b = uniform(it.x);
+    if (bmincore[b] == INT_MAX) return;
+    const int32_t end = uniform(bstart[b + 1]);
+    const int32_t p = it.y + lane_id();
+    const bool pcore = p < end && core[p];
+    if (!__any(pcore)) return;
+    const int32_t u = uniform(bcell[b]);
+    const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
+    T pc[D];
+    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+    const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
+    const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
+    for (int32_t r = r0; r < r1; ++r) {
+        const int2 cs = rcells[r];
+        const int32_t e0 = uniform(cbox[uniform(cs.x)]);
+        if (e0 >= b) break;  // later rows hold later boxes only
+        const int32_t e1 = uniform(min(cbox[uniform(cs.y)], b));
+        for (int32_t b2 = e0; b2 < e1; ++b2) {
+            if (bmincore[b2] == INT_MAX) continue;
+            double gmin2, gmax2;
+            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            const bool act = pcore && gmin2 <= gap2_max;
+            if (!__any(act)) continue;
+            bool hit = false;
+            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+            };
+            auto stop = [&] { return static_cast<bool>(__any(hit)); };
+            if (visit_range<T, D>(soa, ld, core, nullptr, uniform(bstart[b2]),
+                                  uniform(bstart[b2 + 1]), body, stop)) {
+                if (lane_id() == 0) bparent[b] = b2;
+                return;
+            }
+        }
+    }
+}
+
+// root of every box's hook tree -> rsnap (-1 for boxes without cores)
+__global__ __launch_bounds__(kThreads) void k_box_roots(int32_t nbox,
+                                                        const int32_t* __restrict__ bmincore,
+                                                        int32_t* bparent,
+                                                        int32_t* __restrict__ rsnap) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= nbox) return;
+    rsnap[b] = bmincore[b] == INT_MAX ? -1 : uf_find(bparent, b);
+}
+
+// Box-level union: one wave per 64-slot chunk of a box with cores; lanes
+// own the chunk's rows.  Neighbour boxes come from the cell's neighbour rows
+// (cells [a, b) -> boxes [cbox[a], cbox[b])); each unordered box pair is
+// tested by the chunks of its lower box.  A pair is skipped when the boxes
+// are already joined (live roots) or their bounding boxes are farther apart
+// than eps; otherwise the lanes whose row is a core within eps of the other
+// box's bounding box test the other box's cores as they stream through the
+// wave, until the first within-eps pair, which joins the two boxes.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_union(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
+    const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
+    const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
+    const int32_t* __restrict__ rsnap, int32_t* bparent) {
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
+    if (wid >= nitems) return;
+    const int2 it = bitems[wid];
+    const int32_t b = uniform(it.x);
+    if (bmincore[b] == INT_MAX) return;
+    const int32_t sb = uniform(rsnap[b]);
+    const int32_t end = uniform(bstart[b + 1]);
+    const int32_t p = it.y + lane_id();
+    const bool pcore = p < end && core[p];
+    if (!__any(pcore)) return;
+    const int32_t u = uniform(bcell[b]);
+    const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    T pc[D];
+    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
     double lo[D], hi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        lo[j] = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2]));
-        hi[j] = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1]));
+        lo[j] = box_lo<T, D>(bbox, b, j);
+        hi[j] = box_hi<T, D>(bbox, b, j);
     }
-    const int32_t u = bcell[b];
-    const int32_t s0 = bstart[b], s1 = bstart[b + 1];
-    int32_t rb = find_root(bparent, b);
-    for (uint32_t r = roff[u]; r < roff[u + 1]; ++r) {
+    int32_t rb = uniform(uf_find(bparent, b));
+    const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
+    const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
+    for (int32_t r = r0; r < r1; ++r) {
         const int2 cs = rcells[r];
-        const int32_t e0 = cbox[cs.x], e1 = cbox[cs.y];
+        const int32_t e0 = uniform(cbox[uniform(cs.x)]);
+        const int32_t e1 = uniform(cbox[uniform(cs.y)]);
         for (int32_t b2 = max(e0, b + 1); b2 < e1; ++b2) {
-            if (bmincore[b2] == INT_MAX) continue;
-            double gap2 = 0.0;
+            // joined by the hook forest already, or no cores (rsnap -1)
+            const int32_t s2 = uniform(rsnap[b2]);
+            if (s2 == sb || s2 < 0) continue;
+            double gap2 = 0.0, pgap2 = 0.0;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
-                const double lo2 = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b2) * D + j) * 2]));
-                const double hi2 = static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b2) * D + j) * 2 + 1]));
+                const double lo2 = box_lo<T, D>(bbox, b2, j);
+                const double hi2 = box_hi<T, D>(bbox, b2, j);
                 const double g = fmax(0.0, fmax(lo2 - hi[j], lo[j] - hi2));
                 gap2 += g * g;
+                const double v = static_cast<double>(pc[j]);
+                const double h = fmax(0.0, fmax(lo2 - v, v - hi2));
+                pgap2 += h * h;
             }
             if (gap2 > gap2_max) continue;
-            int32_t rb2 = find_root(bparent, b2);
+            const bool act = pcore && pgap2 <= gap2_max;
+            if (!__any(act)) continue;
+            const int32_t rb2 = uniform(uf_find(bparent, b2));
+            rb = uniform(uf_find(bparent, rb));
             if (rb2 == rb) continue;
-            // any within-eps core pair across the two boxes?
             bool hit = false;
-            const int32_t t0 = bstart[b2], t1 = bstart[b2 + 1];
-            for (int32_t i = s0; i < s1 && !hit; ++i) {
-                if (!core[i]) continue;
-                T pi[D];
-#pragma unroll
-                for (int j = 0; j < D; ++j) pi[j] = soa[static_cast<int64_t>(j) * ld + i];
-                for (int32_t q = t0; q < t1; ++q) {
-                    if (!core[q]) continue;
-                    T pq[D];
-#pragma unroll
-                    for (int j = 0; j < D; ++j) pq[j] = soa[static_cast<int64_t>(j) * ld + q];
-                    if (within<D>(pi, pq, eps2)) {
-                        hit = true;
-                        break;
-                    }
-                }
-            }
-            if (hit) {
-                uf_unite(bparent, rb, rb2);
-                rb = find_root(bparent, b);
+            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+            };
+            auto stop = [&] { return static_cast<bool>(__any(hit)); };
+            if (visit_range<T, D>(soa, ld, core, nullptr, uniform(bstart[b2]),
+                                  uniform(bstart[b2 + 1]), body, stop)) {
+                if (lane_id() == 0) uf_unite(bparent, rb, rb2);
+                rb = uniform(uf_find(bparent, b));
             }
         }
     }
@@ -471,10 +532,20 @@ void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rce
     using U = OrdKey<T>;
     hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        bparent, static_cast<int64_t>(a.nbox));
-    hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
-                       s, a.soa, a.ld, a.nbox, a.bstart, a.bcell, a.cbox, roff, rcells,
-                       reinterpret_cast<const U*>(a.bbox), bmincore, core, a.job_of_cell, a.jobs,
-                       bparent);
+    // hook forest, its roots as a snapshot (kept in compmin, which is free
+    // until the union is done), then the union of the pairs the snapshot
+    // does not already join
+    int32_t* rsnap = compmin;
+    hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
+                       a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
+                       a.job_of_cell, a.jobs, bparent);
+    hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                       a.nbox, bmincore, bparent, rsnap);
+    hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
+                       a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
+                       a.job_of_cell, a.jobs, static_cast<const int32_t*>(rsnap), bparent);
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        compmin, INT_MAX, static_cast<int64_t>(a.nbox));
     hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,

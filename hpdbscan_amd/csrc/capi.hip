@@ -439,6 +439,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         launch_box_tight<T, D>(ba, btight, &counters[5], s);
         a.box_of_slot = box_of_slot;
         a.btight = btight;
+        a.cbox = cbox;
+        a.bstart = bstart;
+        a.bbox = bbox;
     }
     tm.mark();  // 3: index done
 
@@ -488,6 +491,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         launch_one_thread_assign<T, D>(a, core, rootslot, amb, namb, f, labels, s);
     }
     tm.mark();  // 6: labels done
+    // labels are complete on return; the timing events are read next
+    if (ctx.timing || (one_thread && hs[3] > 0)) HPD_HIP(hipStreamSynchronize(s));
     ctx.stats.sum_candidates = hs[0];
     ctx.stats.core_points = hs[1];
     ctx.stats.noise_points = hs[2];

@@ -33,11 +33,13 @@
 // own the chunk's rows, the candidate rows stream through the wave in
 // aligned batches fetched with wave-uniform addresses.
 #include <climits>
+#include <type_traits>
 
 #include "common.hpp"
 #include "dbscan.hpp"
 #include "grid.hpp"
 #include "internal.hpp"
+#include "visit.hpp"
 
 namespace hpd {
 
@@ -45,63 +47,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
-
-template <typename T, int D>
-__device__ __forceinline__ void load_row(const T* __restrict__ soa, int64_t ld, int64_t i, T* c) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) c[j] = soa[static_cast<int64_t>(j) * ld + i];
-}
-
-// Candidates are visited in aligned batches of kBatch<T> rows.  Every batch
-// is fetched with wave-uniform addresses before it is used, which lets the
-// compiler issue them as a few wide scalar loads and wait once.
-template <typename T>
-constexpr int kBatch = sizeof(T) == 8 ? 8 : 16;
-
-template <typename U, int B>
-struct alignas(B * sizeof(U) >= 4 ? B * sizeof(U) : 4) Batch {
-    U v[B];
-};
-
-template <typename U, int B>
-__device__ __forceinline__ Batch<U, B> load_batch(const U* __restrict__ base, int64_t a) {
-    return *reinterpret_cast<const Batch<U, B>*>(base + a);
-}
-
-// Visit candidate rows [qb, qe).  body(q, qc, core_q, aux_q) runs per
-// candidate; core_q is flags[q] (1 without flags), aux_q is aux[q] (0 without
-// aux).  stop() is checked after each batch and ends the visit when true.
-// Returns true if stopped.  soa, flags and aux are read up to the next
-// multiple of the batch size past qe (callers pad them to ld rows).
-template <typename T, int D, typename Body, typename Stop>
-__device__ __forceinline__ bool visit_range(const T* __restrict__ soa, int64_t ld,
-                                            const uint8_t* __restrict__ flags,
-                                            const int32_t* __restrict__ aux, int32_t qb, int32_t qe,
-                                            Body&& body, Stop&& stop) {
-    constexpr int B = kBatch<T>;
-    for (int32_t a = qb & ~(B - 1); a < qe; a += B) {
-        Batch<T, B> b[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) b[j] = load_batch<T, B>(soa + static_cast<int64_t>(j) * ld, a);
-        Batch<uint8_t, B> f;
-        if (flags) f = load_batch<uint8_t, B>(flags, a);
-        Batch<int32_t, B> x;
-        if (aux) x = load_batch<int32_t, B>(aux, a);
-        const bool full = a >= qb && a + B <= qe;
-        const int lo = qb > a ? qb - a : 0;
-        const int hi = qe - a < B ? qe - a : B;
-#pragma unroll
-        for (int t = 0; t < B; ++t) {
-            if (!full && (t < lo || t >= hi)) continue;
-            T qc[D];
-#pragma unroll
-            for (int j = 0; j < D; ++j) qc[j] = b[j].v[t];
-            body(a + t, qc, flags ? f.v[t] : uint8_t(1), aux ? x.v[t] : 0);
-        }
-        if (stop()) return true;
-    }
-    return false;
-}
 
 struct Chunk {
     int32_t u;       // occupied cell
@@ -174,6 +119,66 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
         const int2 rg = a.ranges[r];
         done = visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, uniform(rg.x), uniform(rg.y), body,
                                  stop);
+    }
+    const bool is_core = valid && cnt >= min_pts;
+    if (valid) core[p] = is_core ? 1 : 0;
+    const unsigned long long nc = __popcll(ballot(is_core));
+    if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
+}
+
+// Box-mode core counts.  For every neighbour box: lanes whose row is farther
+// than eps from the box's bounding box skip it, lanes whose row is within
+// eps of the box's farthest corner count all its rows, the other lanes
+// test the box's rows one by one.  Exactness: with d the exact distance
+// and gmin2 <= d^2 <= gmax2, the reference's float expression is within a
+// few ulps of d^2, so gmin2 > EPS2 (1 + 1e-5) (+ 1e-30) rules every member
+// out and gmax2 <= EPS2 (1 - 1e-5) rules every member in.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
+                                                         uint8_t* __restrict__ core,
+                                                         unsigned long long* __restrict__ ncore) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
+    const T eps2 = jp.eps2;
+    const uint32_t min_pts = jp.min_pts;
+    const double gap2_max = jp.gap2_max;
+    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+    uint32_t cnt = 0;
+    // rows of a tight box with >= minPts rows are core without counting
+    if (valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = min_pts;
+    bool need = valid && cnt < min_pts;
+    const int32_t center = uniform(a.rcenter[c.u]);
+    for (int32_t i = -1; i < c.r1 - c.r0 && __any(need); ++i) {
+        const int32_t r = i < 0 ? center : c.r0 + i;
+        if (i >= 0 && r == center) continue;
+        const int2 cs = a.rcells[r];
+        const int32_t e0 = uniform(a.cbox[uniform(cs.x)]);
+        const int32_t e1 = uniform(a.cbox[uniform(cs.y)]);
+        for (int32_t b2 = e0; b2 < e1; ++b2) {
+            double gmin2, gmax2;
+            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            const bool near = need && gmin2 <= gap2_max;
+            if (!__any(near)) continue;
+            const int32_t q0 = uniform(a.bstart[b2]), q1 = uniform(a.bstart[b2 + 1]);
+            const bool whole = near && gmax2 <= full2;
+            if (whole) cnt += static_cast<uint32_t>(q1 - q0);
+            const bool part = near && !whole;
+            if (__any(part)) {
+                auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
+                    cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
+                };
+                auto stop = [&] { return static_cast<bool>(__all(!part || cnt >= min_pts)); };
+                visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, q0, q1, body, stop);
+            }
+            need = valid && cnt < min_pts;
+            if (!__any(need)) break;
+        }
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
@@ -406,6 +411,87 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
     }
 }
 
+// Box-mode border labels: every core of a box lies in the box's cluster
+// (bcluster), so a non-core row only needs, per neighbour box whose cluster
+// could change its answer, one within-eps core of that box (or a box within
+// eps as a whole).  A box is skipped for a lane when its cluster is the
+// lane's current one, or larger while the lane has not yet met two
+// clusters; the smallest adjacent cluster and the two-cluster flag come out
+// exact.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a,
+                                                           const uint8_t* __restrict__ core,
+                                                           const int32_t* __restrict__ bcluster,
+                                                           const uint32_t* __restrict__ order,
+                                                           int64_t* __restrict__ labels,
+                                                           unsigned long long* __restrict__ stats,
+                                                           int2* __restrict__ amb_list) {
+    Chunk c;
+    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+    const int64_t first_pos = jp.row_begin;
+    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
+    const int32_t p = c.first + lane_id();
+    const bool valid = p < c.end;
+    const bool active = valid && !core[p];
+    if (!__any(active)) return;
+    T pc[D];
+    load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
+    int32_t best = INT_MAX;
+    bool amb = false;
+    const int32_t center = uniform(a.rcenter[c.u]);
+    for (int32_t i = -1; i < c.r1 - c.r0; ++i) {
+        const int32_t r = i < 0 ? center : c.r0 + i;
+        if (i >= 0 && r == center) continue;
+        const int2 cs = a.rcells[r];
+        const int32_t e0 = uniform(a.cbox[uniform(cs.x)]);
+        const int32_t e1 = uniform(a.cbox[uniform(cs.y)]);
+        for (int32_t b2 = e0; b2 < e1; ++b2) {
+            const int32_t cl = uniform(bcluster[b2]);
+            if (cl < 0) continue;  // no cores
+            const bool want = active && cl != best && (cl < best || !amb);
+            if (!__any(want)) continue;
+            double gmin2, gmax2;
+            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            const bool near = want && gmin2 <= gap2_max;
+            if (!__any(near)) continue;
+            bool hit = near && gmax2 <= full2;
+            const bool part = near && !hit;
+            if (__any(part)) {
+                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                    if (part && !hit && cq && within<D>(pc, qc, eps2)) hit = true;
+                };
+                auto stop = [&] { return static_cast<bool>(__all(!part || hit)); };
+                visit_range<T, D>(a.soa, a.ld, core, nullptr, uniform(a.bstart[b2]),
+                                  uniform(a.bstart[b2 + 1]), body, stop);
+            }
+            if (hit) {
+                if (best != INT_MAX) amb = true;
+                best = min(best, cl);
+            }
+        }
+    }
+    if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
+    const bool is_amb = active && amb;
+    const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
+    const uint64_t amb_mask = ballot(is_amb);
+    const unsigned long long namb = __popcll(amb_mask);
+    unsigned long long base = 0;
+    if (lane_id() == 0) {
+        if (noise) atomicAdd(&stats[0], noise);
+        if (namb) base = atomicAdd(&stats[1], namb);
+    }
+    if (amb_list && namb) {
+        base = __shfl(base, 0, 64);
+        if (is_amb)
+            amb_list[base + static_cast<unsigned long long>(__popcll(amb_mask & lanemask_lt()))] =
+                make_int2(p, c.u);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The reference's border choice at one thread (HPD_BORDER_ONE_THREAD).
 //
@@ -545,6 +631,14 @@ inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n +
 template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s) {
     if (a.nitems <= 0) return;
+    if constexpr (std::is_floating_point<T>::value) {
+        if (a.bbox) {
+            hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
+                               dim3(kThreads), 0, s, a, core, ncore);
+            HPD_HIP(hipGetLastError());
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_core<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)), dim3(kThreads),
                        0, s, a, core, ncore);
     HPD_HIP(hipGetLastError());
@@ -584,6 +678,14 @@ void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
     hipLaunchKernelGGL(k_label_cores<T>, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems,
                        a.ustart, a.roff, a.jobs, a.job_of_cell, core, a.box_of_slot, bcluster,
                        parent, compmin, a.inrow, labels, rootslot, cmin, cmax);
+    if constexpr (std::is_floating_point<T>::value) {
+        if (bcluster && a.bbox) {
+            hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
+                               bcluster, a.inrow, labels, stats, amb_list);
+            HPD_HIP(hipGetLastError());
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
                        static_cast<const int32_t*>(rootslot), static_cast<const int32_t*>(cmin),
                        static_cast<const int32_t*>(cmax), a.inrow, labels, stats, amb_list);

@@ -29,8 +29,16 @@ struct DbscanArgs {
     const uint32_t* inrow = nullptr;        // input row per slot
     const int32_t* box_of_slot = nullptr;   // box per slot (null: no boxes)
     const uint8_t* btight = nullptr;        // box flags (bit 1: all rows core)
+    // box tables (null without boxes): boxes of occupied cell u are
+    // [cbox[u], cbox[u+1]), rows of box b are slots [bstart[b], bstart[b+1]),
+    // bbox holds ordered-key min / max per box and dimension
+    const int32_t* cbox = nullptr;
+    const int32_t* bstart = nullptr;
+    const void* bbox = nullptr;
 };
 
+// core flag of every slot; with box tables the candidates are visited box
+// by box (boxes beyond eps skipped, boxes within eps counted whole)
 template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s);
 
@@ -45,7 +53,8 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 
 // Labels: core slots get -(cluster + 1), where cluster comes from bcluster
 // (box mode, bcluster != null) or compmin[find(parent, slot)]; then border
-// and noise labels.  Labels go to input order through a.inrow.
+// and noise labels (in box mode box by box: all cores of a box share its
+// cluster).  Labels go to input order through a.inrow.
 // rootslot: ld entries.  stats[0] += noise rows, stats[1] += ambiguous
 // border rows; when amb_list is given it receives (slot, cell) of each
 // ambiguous border row (capacity n).

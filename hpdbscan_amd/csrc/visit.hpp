@@ -1,0 +1,120 @@
+// Streaming candidate rows through a wave: the rows are read in aligned
+// batches with wave-uniform addresses (a few wide scalar loads per batch),
+// every lane tests its own row against each candidate of the batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace hpd {
+
+template <typename T, int D>
+__device__ __forceinline__ void load_row(const T* __restrict__ soa, int64_t ld, int64_t i, T* c) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) c[j] = soa[static_cast<int64_t>(j) * ld + i];
+}
+
+// Candidates are visited in aligned batches of kBatch<T> rows.  Every batch
+// is fetched with wave-uniform addresses before it is used, which lets the
+// compiler issue them as a few wide scalar loads and wait once.
+template <typename T>
+constexpr int kBatch = sizeof(T) == 8 ? 8 : 16;
+
+template <typename U, int B>
+struct alignas(B * sizeof(U) >= 4 ? B * sizeof(U) : 4) Batch {
+    U v[B];
+};
+
+template <typename U, int B>
+__device__ __forceinline__ Batch<U, B> load_batch(const U* __restrict__ base, int64_t a) {
+    return *reinterpret_cast<const Batch<U, B>*>(base + a);
+}
+
+// Visit candidate rows [qb, qe).  body(q, qc, core_q, aux_q) runs per
+// candidate; core_q is flags[q] (1 without flags), aux_q is aux[q] (0 without
+// aux).  stop() is checked after each batch and ends the visit when true.
+// Returns true if stopped.  soa, flags and aux are read up to the next
+// multiple of the batch size past qe (callers pad them to ld rows).
+template <typename T, int D, typename Body, typename Stop>
+__device__ __forceinline__ bool visit_range(const T* __restrict__ soa, int64_t ld,
+                                            const uint8_t* __restrict__ flags,
+                                            const int32_t* __restrict__ aux, int32_t qb, int32_t qe,
+                                            Body&& body, Stop&& stop) {
+    constexpr int B = kBatch<T>;
+    for (int32_t a = qb & ~(B - 1); a < qe; a += B) {
+        Batch<T, B> b[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) b[j] = load_batch<T, B>(soa + static_cast<int64_t>(j) * ld, a);
+        Batch<uint8_t, B> f;
+        if (flags) f = load_batch<uint8_t, B>(flags, a);
+        Batch<int32_t, B> x;
+        if (aux) x = load_batch<int32_t, B>(aux, a);
+        const bool full = a >= qb && a + B <= qe;
+        const int lo = qb > a ? qb - a : 0;
+        const int hi = qe - a < B ? qe - a : B;
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            if (!full && (t < lo || t >= hi)) continue;
+            T qc[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) qc[j] = b[j].v[t];
+            body(a + t, qc, flags ? f.v[t] : uint8_t(1), aux ? x.v[t] : 0);
+        }
+        if (stop()) return true;
+    }
+    return false;
+}
+
+// order-preserving integer images of floating-point values (for atomics)
+__device__ __forceinline__ uint32_t ord_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ unsigned long long ord_key(double f) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(f));
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord_val(unsigned long long k) {
+    return __longlong_as_double(static_cast<long long>((k & 0x8000000000000000ull)
+                                                           ? (k & 0x7fffffffffffffffull)
+                                                           : ~k));
+}
+
+template <typename T>
+using OrdKey = std::conditional_t<sizeof(T) == 8, unsigned long long, uint32_t>;
+
+template <typename T, int D>
+__device__ __forceinline__ double box_lo(const OrdKey<T>* __restrict__ bbox, int32_t b, int j) {
+    return static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2]));
+}
+template <typename T, int D>
+__device__ __forceinline__ double box_hi(const OrdKey<T>* __restrict__ bbox, int32_t b, int j) {
+    return static_cast<double>(ord_val(bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1]));
+}
+
+// Squared distances from a row to a box's bounding box, in double: gmin2
+// to its nearest point, gmax2 to its farthest corner.  Every member q of the
+// box satisfies gmin2 <= |p - q|^2 <= gmax2.
+template <typename T, int D>
+__device__ __forceinline__ void point_box(const OrdKey<T>* __restrict__ bbox, int32_t b,
+                                          const T* pc, double& gmin2, double& gmax2) {
+    gmin2 = 0.0;
+    gmax2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double lo = box_lo<T, D>(bbox, b, j);
+        const double hi = box_hi<T, D>(bbox, b, j);
+        const double v = static_cast<double>(pc[j]);
+        const double g = fmax(0.0, fmax(lo - v, v - hi));
+        const double f = fmax(v - lo, hi - v);
+        gmin2 += g * g;
+        gmax2 += f * f;
+    }
+}
+
+}  // namespace hpd
