@@ -53,10 +53,12 @@ class RunStats(ctypes.Structure):
         ("ms_union", ctypes.c_double),
         ("ms_border", ctypes.c_double),
         ("ms_total", ctypes.c_double),
-        ("core_launches", ctypes.c_int32),
-        ("ms_core_kernel", ctypes.c_double),
+        ("ms_hook_kernel", ctypes.c_double),
+        ("ms_union_kernel", ctypes.c_double),
+        ("ms_border_kernel", ctypes.c_double),
         ("boxes", ctypes.c_uint64),
         ("box_union", ctypes.c_int32),
+        ("neighbor_rows", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

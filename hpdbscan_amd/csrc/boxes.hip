@@ -526,32 +526,41 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
 }
 
 template <typename T, int D>
-void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                      const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
-                      int32_t* compmin, int32_t* bcluster, hipStream_t s) {
+void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
+                     int32_t* rsnap, hipStream_t s) {
     using U = OrdKey<T>;
     hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        bparent, static_cast<int64_t>(a.nbox));
-    // hook forest, its roots as a snapshot (kept in compmin, which is free
-    // until the union is done), then the union of the pairs the snapshot
-    // does not already join
-    int32_t* rsnap = compmin;
     hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
                        a.job_of_cell, a.jobs, bparent);
     hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        a.nbox, bmincore, bparent, rsnap);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
+                     int32_t* bparent, hipStream_t s) {
+    using U = OrdKey<T>;
     hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
-                       a.job_of_cell, a.jobs, static_cast<const int32_t*>(rsnap), bparent);
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                       compmin, INT_MAX, static_cast<int64_t>(a.nbox));
-    hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                       a.nbox, bmincore, bparent, compmin);
-    hipLaunchKernelGGL(k_box_cluster, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                       a.nbox, bmincore, bparent, static_cast<const int32_t*>(compmin), bcluster);
+                       a.job_of_cell, a.jobs, rsnap, bparent);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
+                         int32_t* compmin, int32_t* bcluster, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       compmin, INT_MAX, static_cast<int64_t>(nbox));
+    hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       nbox, bmincore, bparent, compmin);
+    hipLaunchKernelGGL(k_box_cluster, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       nbox, bmincore, bparent, static_cast<const int32_t*>(compmin), bcluster);
     HPD_HIP(hipGetLastError());
 }
 
@@ -570,9 +579,12 @@ void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rce
                                           hipStream_t);                                           \
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
-    template void launch_box_union<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,         \
-                                         const int32_t*, const uint8_t*, int32_t*, int32_t*,      \
-                                         int32_t*, hipStream_t);
+    template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
+                                        const int32_t*, const uint8_t*, int32_t*, int32_t*,       \
+                                        hipStream_t);                                             \
+    template void launch_box_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
+                                        const int32_t*, const uint8_t*, const int32_t*, int32_t*, \
+                                        hipStream_t);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \
     HPD_INST_BOX_KEYS(T, 2)   \

@@ -66,11 +66,22 @@ void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bminco
 template <typename T, int D>
 void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s);
 
-// components of the boxes with cores; bcluster[b] = smallest core position
-// of b's component (-1 without cores)
+// Components of the boxes with cores, in three launches:
+//   launch_box_hook      hook forest (every box points at some earlier box
+//                        joined to it) and its roots -> rsnap (nbox entries,
+//                        -1 for boxes without cores)
+//   launch_box_join      union of the box pairs the snapshot does not join
+//   launch_box_clusters  bcluster[b] = smallest core position of b's
+//                        component (-1 without cores); uses compmin (nbox)
 template <typename T, int D>
-void launch_box_union(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                      const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
-                      int32_t* compmin, int32_t* bcluster, hipStream_t s);
+void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
+                     int32_t* rsnap, hipStream_t s);
+template <typename T, int D>
+void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
+                     int32_t* bparent, hipStream_t s);
+void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
+                         int32_t* compmin, int32_t* bcluster, hipStream_t s);
 
 }  // namespace hpd

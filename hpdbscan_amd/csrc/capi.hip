@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -185,6 +186,13 @@ struct Context {
     Arena aux;
     Arena io;  // input rows and labels of the host-memory entry point
     int border_rule = HPD_BORDER_MIN_CLUSTER;
+    // cells with at most this many rows are handled one thread per row in
+    // the core count / border pass (box mode); HPD_SPARSE_CORE and
+    // HPD_SPARSE_BORDER override them.  Measured on the 10M x 3-D workload:
+    // the border pass gains 7x at 4-16, the core count loses a little at
+    // any threshold (fringe rows next to dense boxes serialise per thread).
+    int32_t sparse_core = 0;
+    int32_t sparse_border = 8;
     bool timing = false;
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
@@ -203,32 +211,49 @@ struct DeviceGuard {
     }
 };
 
+// Named points of a run at which HIP events are recorded on the run's
+// stream (when timing is on).
+enum Mark : int {
+    M_START,
+    M_GRID,        // min/max and cell ids done
+    M_SORT,        // both sorts and the gather done
+    M_INDEX,       // cells, neighbour rows, boxes done
+    M_CORE,        // core flags done
+    M_HOOK0,       // box union: hook forest starts
+    M_HOOK1,       // box union: hook forest and its roots done
+    M_JOIN1,       // box union: union kernel done
+    M_UNION,       // clusters of all cores known
+    M_BORDER0,     // core labels written, border pass starts
+    M_LABELS,      // all labels written
+    M_COUNT
+};
+
 class PhaseTimer {
 public:
     PhaseTimer(Context& c, hipStream_t s) : ctx_(c), s_(s) {}
-    void mark() {
+    void mark(Mark m) {
         if (!ctx_.timing) return;
-        if (idx_ >= ctx_.events.size()) {
+        while (ctx_.events.size() < M_COUNT) {
             hipEvent_t e;
             HPD_HIP(hipEventCreate(&e));
             ctx_.events.push_back(e);
         }
-        HPD_HIP(hipEventRecord(ctx_.events[idx_], s_));
-        ++idx_;
+        HPD_HIP(hipEventRecord(ctx_.events[m], s_));
+        seen_[m] = true;
     }
-    // milliseconds between mark i and mark j (after a stream sync)
-    double ms(size_t i, size_t j) const {
-        if (!ctx_.timing || j >= idx_) return 0.0;
+    // milliseconds between two recorded marks (after a stream sync); 0 if
+    // either was not recorded in this run
+    double ms(Mark i, Mark j) const {
+        if (!ctx_.timing || !seen_[i] || !seen_[j]) return 0.0;
         float v = 0.0f;
         HPD_HIP(hipEventElapsedTime(&v, ctx_.events[i], ctx_.events[j]));
         return static_cast<double>(v);
     }
-    size_t count() const { return idx_; }
 
 private:
     Context& ctx_;
     hipStream_t s_;
-    size_t idx_ = 0;
+    bool seen_[M_COUNT] = {};
 };
 
 // --------------------------------------------------------------------------
@@ -300,7 +325,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
 
     // cell ids (spatial_index.h:144-165), offset by the job's key base
     launch_cell_ids<T, D, K>(pts, n, jobs, njobs, keys_a, vals_a, s);
-    tm.mark();  // 1: grid done
+    tm.mark(M_GRID);
 
     // stable sort by cell (spatial_index.h:167-216): position -> input row;
     // jobs stay contiguous and in order because their key ranges are
@@ -368,7 +393,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     } else {
         launch_gather_soa<T, D>(pts, n, inrow_of_pos, soa, ld, s);
     }
-    tm.mark();  // 2: sort done
+    tm.mark(M_SORT);
 
     // neighbour ranges (spatial_index.h:682-737) and work items
     launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, &dstats[0], s);
@@ -441,12 +466,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         a.btight = btight;
         a.cbox = cbox;
         a.bstart = bstart;
+        a.bcell = bcell;
         a.bbox = bbox;
+        a.sparse_core = ctx.sparse_core;
+        a.sparse_border = ctx.sparse_border;
     }
-    tm.mark();  // 3: index done
+    tm.mark(M_INDEX);
 
     launch_core<T, D>(a, core, &dstats[1], s);
-    tm.mark();  // 4: core done
+    tm.mark(M_CORE);
 
     if constexpr (kFloat) {
         HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -455,16 +483,25 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     }
     if (box_union) {
         launch_box_reduce<T, D>(ba, core, bmincore, s);
-        launch_box_union<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, bcluster, s);
+        // compmin holds the hook forest's roots until the clusters are known
+        tm.mark(M_HOOK0);
+        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, s);
+        tm.mark(M_HOOK1);
+        launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
+        tm.mark(M_JOIN1);
+        launch_box_clusters(ba.nbox, bmincore, bparent, compmin, bcluster, s);
     } else {
         if (!compmin) compmin = A.take_n<int32_t>(nn);
         launch_union_points<T, D>(a, core, parent, rsnap, cmin, cmax, compmin, s);
     }
-    tm.mark();  // 5: union done
+    tm.mark(M_UNION);
     const bool one_thread = ctx.border_rule == HPD_BORDER_ONE_THREAD;
     int2* amb = one_thread ? A.take_n<int2>(nn) : nullptr;
-    launch_labels<T, D>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
-                        cmax, labels, &dstats[2], amb, s);
+    launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
+                          cmax, labels, s);
+    tm.mark(M_BORDER0);
+    launch_border<T, D>(a, core, box_union ? bcluster : nullptr, rootslot, cmin, cmax, labels,
+                        &dstats[2], amb, s);
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
     HPD_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -490,7 +527,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         }
         launch_one_thread_assign<T, D>(a, core, rootslot, amb, namb, f, labels, s);
     }
-    tm.mark();  // 6: labels done
+    tm.mark(M_LABELS);
     // labels are complete on return; the timing events are read next
     if (ctx.timing || (one_thread && hs[3] > 0)) HPD_HIP(hipStreamSynchronize(s));
     ctx.stats.sum_candidates = hs[0];
@@ -498,6 +535,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     ctx.stats.noise_points = hs[2];
     ctx.stats.ambiguous_borders = hs[3];
     ctx.stats.occupied_cells = static_cast<uint64_t>(m);
+    ctx.stats.neighbor_rows = total_ranges;
     ctx.stats.boxes = static_cast<uint64_t>(nbox);
     ctx.stats.box_union = box_union ? 1 : 0;
 }
@@ -509,7 +547,7 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
                       int32_t njobs, const float* eps, const uint64_t* min_pts, int64_t* labels,
                       hipStream_t s) {
     PhaseTimer tm(ctx, s);
-    tm.mark();  // 0: start
+    tm.mark(M_START);
     Arena& A = ctx.main;
     A.reserve(workspace_bytes<T, D>(n, njobs, ctx.border_rule == HPD_BORDER_ONE_THREAD));
 
@@ -590,15 +628,16 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
     else
         run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, labels, s, tm);
     if (ctx.timing) {
-        ctx.stats.ms_grid = tm.ms(0, 1);
-        ctx.stats.ms_sort = tm.ms(1, 2);
-        ctx.stats.ms_index = tm.ms(2, 3);
-        ctx.stats.ms_core = tm.ms(3, 4);
-        ctx.stats.ms_union = tm.ms(4, 5);
-        ctx.stats.ms_border = tm.ms(5, 6);
-        ctx.stats.ms_total = tm.ms(0, 6);
-        ctx.stats.core_launches = 1;
-        ctx.stats.ms_core_kernel = ctx.stats.ms_core;
+        ctx.stats.ms_grid = tm.ms(M_START, M_GRID);
+        ctx.stats.ms_sort = tm.ms(M_GRID, M_SORT);
+        ctx.stats.ms_index = tm.ms(M_SORT, M_INDEX);
+        ctx.stats.ms_core = tm.ms(M_INDEX, M_CORE);
+        ctx.stats.ms_union = tm.ms(M_CORE, M_UNION);
+        ctx.stats.ms_border = tm.ms(M_UNION, M_LABELS);
+        ctx.stats.ms_total = tm.ms(M_START, M_LABELS);
+        ctx.stats.ms_hook_kernel = tm.ms(M_HOOK0, M_HOOK1);
+        ctx.stats.ms_union_kernel = tm.ms(M_HOOK1, M_JOIN1);
+        ctx.stats.ms_border_kernel = tm.ms(M_BORDER0, M_LABELS);
     }
 }
 
@@ -719,6 +758,10 @@ int hpd_context_create(int device, hpd_context** out) {
         auto* c = new hpd_context();
         try {
             c->impl.device = device;
+            if (const char* e = std::getenv("HPD_SPARSE_CORE"))
+                c->impl.sparse_core = std::max(0, std::atoi(e));
+            if (const char* e = std::getenv("HPD_SPARSE_BORDER"))
+                c->impl.sparse_border = std::max(0, std::atoi(e));
             hpd::DeviceGuard g(device);
             HPD_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
             void* hs = nullptr;

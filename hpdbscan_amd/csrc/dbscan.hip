@@ -139,6 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
                                                          unsigned long long* __restrict__ ncore) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    if (c.end - uniform(a.ustart[c.u]) <= a.sparse_core) return;  // thread-per-row kernel
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const uint32_t min_pts = jp.min_pts;
@@ -182,6 +183,58 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
+    const unsigned long long nc = __popcll(ballot(is_core));
+    if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
+}
+
+// Thread-per-row core counts for the rows of sparse cells (at most
+// a.sparse_core rows): the same box-by-box count as k_core_boxes, with every
+// thread walking its own row's neighbourhood.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
+                                                          uint8_t* __restrict__ core,
+                                                          unsigned long long* __restrict__ ncore) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool is_core = false;
+    if (s < a.n) {
+        const int32_t p = static_cast<int32_t>(s);
+        const int32_t b = a.box_of_slot[p];
+        const int32_t u = a.bcell[b];
+        if (a.ustart[u + 1] - a.ustart[u] <= a.sparse_core) {
+            const JobParams<T>& jp = a.jobs[a.job_of_cell[u]];
+            const uint32_t min_pts = jp.min_pts;
+            uint32_t cnt = (a.btight[b] & 2) ? min_pts : 0u;
+            if (cnt < min_pts) {
+                const T eps2 = jp.eps2;
+                const double gap2_max = jp.gap2_max;
+                const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+                const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
+                T pc[D];
+                load_row<T, D>(a.soa, a.ld, p, pc);
+                for (uint32_t r = a.roff[u]; r < a.roff[u + 1] && cnt < min_pts; ++r) {
+                    const int2 cs = a.rcells[r];
+                    const int32_t e1 = a.cbox[cs.y];
+                    for (int32_t b2 = a.cbox[cs.x]; b2 < e1 && cnt < min_pts; ++b2) {
+                        double gmin2, gmax2;
+                        point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+                        if (gmin2 > gap2_max) continue;
+                        const int32_t q0 = a.bstart[b2], q1 = a.bstart[b2 + 1];
+                        if (gmax2 <= full2) {
+                            cnt += static_cast<uint32_t>(q1 - q0);
+                            continue;
+                        }
+                        for (int32_t q = q0; q < q1 && cnt < min_pts; ++q) {
+                            T qc[D];
+                            load_row<T, D>(a.soa, a.ld, q, qc);
+                            cnt += within<D>(pc, qc, eps2) ? 1u : 0u;
+                        }
+                    }
+                }
+            }
+            is_core = cnt >= min_pts;
+            core[p] = is_core ? 1 : 0;
+        }
+    }
     const unsigned long long nc = __popcll(ballot(is_core));
     if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
 }
@@ -428,6 +481,7 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
                                                            int2* __restrict__ amb_list) {
     Chunk c;
     if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
+    if (c.end - uniform(a.ustart[c.u]) <= a.sparse_border) return;  // thread-per-row kernel
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
@@ -489,6 +543,74 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
         if (is_amb)
             amb_list[base + static_cast<unsigned long long>(__popcll(amb_mask & lanemask_lt()))] =
                 make_int2(p, c.u);
+    }
+}
+
+// Thread-per-row border labels for the non-core rows of sparse cells; same
+// box-by-box search as k_border_boxes.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> a,
+                                                            const uint8_t* __restrict__ core,
+                                                            const int32_t* __restrict__ bcluster,
+                                                            int64_t* __restrict__ labels,
+                                                            unsigned long long* __restrict__ stats,
+                                                            int2* __restrict__ amb_list) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool active = false, amb = false;
+    int32_t best = INT_MAX, p = 0, u = 0;
+    if (s < a.n) {
+        p = static_cast<int32_t>(s);
+        u = a.bcell[a.box_of_slot[p]];
+        active = !core[p] && a.ustart[u + 1] - a.ustart[u] <= a.sparse_border;
+    }
+    if (active) {
+        const JobParams<T>& jp = a.jobs[a.job_of_cell[u]];
+        const T eps2 = jp.eps2;
+        const double gap2_max = jp.gap2_max;
+        const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+        const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
+        T pc[D];
+        load_row<T, D>(a.soa, a.ld, p, pc);
+        for (uint32_t r = a.roff[u]; r < a.roff[u + 1]; ++r) {
+            const int2 cs = a.rcells[r];
+            const int32_t e1 = a.cbox[cs.y];
+            for (int32_t b2 = a.cbox[cs.x]; b2 < e1; ++b2) {
+                const int32_t cl = bcluster[b2];
+                if (cl < 0 || cl == best || (cl > best && amb)) continue;
+                double gmin2, gmax2;
+                point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+                if (gmin2 > gap2_max) continue;
+                bool hit = gmax2 <= full2;
+                const int32_t q1 = a.bstart[b2 + 1];
+                for (int32_t q = a.bstart[b2]; q < q1 && !hit; ++q) {
+                    if (!core[q]) continue;
+                    T qc[D];
+                    load_row<T, D>(a.soa, a.ld, q, qc);
+                    hit = within<D>(pc, qc, eps2);
+                }
+                if (hit) {
+                    if (best != INT_MAX) amb = true;
+                    best = min(best, cl);
+                }
+            }
+        }
+        labels[a.inrow[p]] =
+            best == INT_MAX ? 0 : static_cast<int64_t>(best) - jp.row_begin + 1;
+    }
+    const bool is_amb = active && amb;
+    const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
+    const uint64_t amb_mask = ballot(is_amb);
+    const unsigned long long namb = __popcll(amb_mask);
+    unsigned long long base = 0;
+    if (lane_id() == 0) {
+        if (noise) atomicAdd(&stats[0], noise);
+        if (namb) base = atomicAdd(&stats[1], namb);
+    }
+    if (amb_list && namb) {
+        base = __shfl(base, 0, 64);
+        if (is_amb)
+            amb_list[base + static_cast<unsigned long long>(__popcll(amb_mask & lanemask_lt()))] =
+                make_int2(p, u);
     }
 }
 
@@ -635,6 +757,9 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
         if (a.bbox) {
             hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
                                dim3(kThreads), 0, s, a, core, ncore);
+            if (a.sparse_core > 0)
+                hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(a.n, kThreads)),
+                                   dim3(kThreads), 0, s, a, core, ncore);
             HPD_HIP(hipGetLastError());
             return;
         }
@@ -666,11 +791,10 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
     HPD_HIP(hipGetLastError());
 }
 
-template <typename T, int D>
-void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
-                   int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                   int32_t* cmax, int64_t* labels, unsigned long long* stats, int2* amb_list,
-                   hipStream_t s) {
+template <typename T>
+void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                        int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
+                        int32_t* cmax, int64_t* labels, hipStream_t s) {
     if (a.n <= 0) return;
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
@@ -678,17 +802,29 @@ void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
     hipLaunchKernelGGL(k_label_cores<T>, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems,
                        a.ustart, a.roff, a.jobs, a.job_of_cell, core, a.box_of_slot, bcluster,
                        parent, compmin, a.inrow, labels, rootslot, cmin, cmax);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                   const int32_t* rootslot, const int32_t* cmin, const int32_t* cmax,
+                   int64_t* labels, unsigned long long* stats, int2* amb_list, hipStream_t s) {
+    if (a.n <= 0) return;
+    const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     if constexpr (std::is_floating_point<T>::value) {
         if (bcluster && a.bbox) {
             hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
                                bcluster, a.inrow, labels, stats, amb_list);
+            if (a.sparse_border > 0)
+                hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.n, kThreads)),
+                                   dim3(kThreads), 0, s, a, core, bcluster, labels, stats,
+                                   amb_list);
             HPD_HIP(hipGetLastError());
             return;
         }
     }
-    hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
-                       static_cast<const int32_t*>(rootslot), static_cast<const int32_t*>(cmin),
-                       static_cast<const int32_t*>(cmax), a.inrow, labels, stats, amb_list);
+    hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core, rootslot, cmin,
+                       cmax, a.inrow, labels, stats, amb_list);
     HPD_HIP(hipGetLastError());
 }
 
@@ -726,9 +862,9 @@ void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const
                                     hipStream_t);                                                 \
     template void launch_union_points<T, D>(const DbscanArgs<T>&, const uint8_t*, int32_t*,       \
                                             int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t); \
-    template void launch_labels<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \
-                                      int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*,     \
-                                      int64_t*, unsigned long long*, int2*, hipStream_t);         \
+    template void launch_border<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \
+                                      const int32_t*, const int32_t*, const int32_t*, int64_t*,   \
+                                      unsigned long long*, int2*, hipStream_t);                   \
     template void launch_one_thread_mark<T, D>(const DbscanArgs<T>&, const uint8_t*,              \
                                                const int32_t*, const int2*, int32_t, uint8_t*,    \
                                                int32_t*, hipStream_t);                            \
@@ -749,5 +885,11 @@ void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const
     HPD_INST_DBSCAN(T, 8)
 
 HPD_FOR_EACH_DTYPE(HPD_INST_DBSCAN_ALL_D)
+
+#define HPD_INST_LABEL_CORES(T)                                                                \
+    template void launch_label_cores<T>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,  \
+                                        int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, \
+                                        int64_t*, hipStream_t);
+HPD_FOR_EACH_DTYPE(HPD_INST_LABEL_CORES)
 
 }  // namespace hpd

@@ -34,7 +34,13 @@ struct DbscanArgs {
     // bbox holds ordered-key min / max per box and dimension
     const int32_t* cbox = nullptr;
     const int32_t* bstart = nullptr;
+    const int32_t* bcell = nullptr;         // occupied cell of every box
     const void* bbox = nullptr;
+    // box mode: rows of cells with at most this many rows are handled one
+    // thread per row instead of one wave per 64-row chunk (0: never), in
+    // the core count and in the border pass
+    int32_t sparse_core = 0;
+    int32_t sparse_border = 0;
 };
 
 // core flag of every slot; with box tables the candidates are visited box
@@ -51,18 +57,24 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
                          int32_t* rsnap, int32_t* cmin, int32_t* cmax, int32_t* compmin,
                          hipStream_t s);
 
-// Labels: core slots get -(cluster + 1), where cluster comes from bcluster
-// (box mode, bcluster != null) or compmin[find(parent, slot)]; then border
-// and noise labels (in box mode box by box: all cores of a box share its
-// cluster).  Labels go to input order through a.inrow.
-// rootslot: ld entries.  stats[0] += noise rows, stats[1] += ambiguous
-// border rows; when amb_list is given it receives (slot, cell) of each
-// ambiguous border row (capacity n).
+// Core labels: core slots get -(cluster + 1), where cluster comes from
+// bcluster (box mode, bcluster != null) or compmin[find(parent, slot)];
+// rootslot[slot] = cluster, cmin / cmax = smallest / largest cluster per
+// occupied cell.  Labels go to input order through a.inrow.
+// rootslot: ld entries.
+template <typename T>
+void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                        int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
+                        int32_t* cmax, int64_t* labels, hipStream_t s);
+
+// Border and noise labels of the non-core rows (in box mode box by box:
+// all cores of a box share its cluster).  stats[0] += noise rows,
+// stats[1] += ambiguous border rows; when amb_list is given it receives
+// (slot, cell) of each ambiguous border row (capacity n).
 template <typename T, int D>
-void launch_labels(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
-                   int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                   int32_t* cmax, int64_t* labels, unsigned long long* stats, int2* amb_list,
-                   hipStream_t s);
+void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
+                   const int32_t* rootslot, const int32_t* cmin, const int32_t* cmax,
+                   int64_t* labels, unsigned long long* stats, int2* amb_list, hipStream_t s);
 
 // The reference's one-thread border choice for the namb ambiguous rows:
 // mark the touched clusters (involved, indexed by cluster position, zeroed

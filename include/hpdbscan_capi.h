@@ -85,10 +85,12 @@ typedef struct hpd_run_stats {
     double ms_union;             /* core-core union-find */
     double ms_border;            /* roots, border labels, scatter */
     double ms_total;             /* first to last event */
-    int32_t core_launches;       /* launches of the core-count kernel */
-    double ms_core_kernel;       /* summed duration of those launches */
+    double ms_hook_kernel;       /* box union: hook forest and its roots */
+    double ms_union_kernel;      /* box union: union of the remaining pairs */
+    double ms_border_kernel;     /* border / noise labels of non-core rows */
     uint64_t boxes;              /* sub-boxes of the occupied cells */
     int32_t box_union;           /* 1 if the union ran on sub-boxes */
+    uint64_t neighbor_rows;      /* [first, end) slot ranges of all cells */
 } hpd_run_stats;
 
 /* Thread-local message describing the last failure ("" if none). */
