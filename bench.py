@@ -354,7 +354,8 @@ def main() -> None:
                     "summed time of the core, hook, union and border kernels, which skip tests "
                     "that bounding boxes decide"},
         "stats": {k: int(st[k]) for k in ("occupied_cells", "core_points", "noise_points",
-                                          "ambiguous_borders", "sum_candidates")},
+                                          "ambiguous_borders", "sum_candidates", "boxes",
+                                          "neighbor_rows", "open_box_pairs")},
     }
 
     if not a.no_cpu_baseline:

@@ -59,6 +59,7 @@ class RunStats(ctypes.Structure):
         ("boxes", ctypes.c_uint64),
         ("box_union", ctypes.c_int32),
         ("neighbor_rows", ctypes.c_uint64),
+        ("open_box_pairs", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
     const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
-    int32_t* __restrict__ bparent) {
+    const int32_t* __restrict__ rcenter, int32_t* __restrict__ bparent) {
     const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                 static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return;
@@ -259,10 +259,15 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
     const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
     const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
-    for (int32_t r = r0; r < r1; ++r) {
+    // the cell's own row first: the box's nearest neighbours are there, so
+    // the first hit comes soonest
+    const int32_t center = uniform(rcenter[u]);
+    for (int32_t i = -1; i < r1 - r0; ++i) {
+        const int32_t r = i < 0 ? center : r0 + i;
+        if (i >= 0 && r == center) continue;
         const int2 cs = rcells[r];
         const int32_t e0 = uniform(cbox[uniform(cs.x)]);
-        if (e0 >= b) break;  // later rows hold later boxes only
+        if (e0 >= b) continue;  // only boxes before b
         const int32_t e1 = uniform(min(cbox[uniform(cs.y)], b));
         for (int32_t b2 = e0; b2 < e1; ++b2) {
             if (bmincore[b2] == INT_MAX) continue;
@@ -280,6 +285,111 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
                 if (lane_id() == 0) bparent[b] = b2;
                 return;
             }
+        }
+    }
+}
+
+// Box pairs the hook forest leaves open: one thread per box b finds every
+// later box b2 with cores in a different hook tree whose bounding box is
+// within eps of b's.  Count pass (list == nullptr): cnt[b] = their number;
+// fill pass: the b2 go to list[off[b] ..].
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_scan(
+    int32_t nbox, const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
+    const uint32_t* __restrict__ roff, const int2* __restrict__ rcells,
+    const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ job_of_cell,
+    const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ rsnap,
+    uint32_t* __restrict__ cnt, int32_t* __restrict__ list, unsigned long long* __restrict__ total) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int32_t sb = b < nbox ? rsnap[b] : -1;
+    uint32_t k = (list && b < nbox) ? cnt[b] : 0u;
+    if (sb >= 0) {
+        const int32_t u = bcell[b];
+        const double gap2_max = jobs[job_of_cell[u]].gap2_max;
+        double lo[D], hi[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            lo[j] = box_lo<T, D>(bbox, b, j);
+            hi[j] = box_hi<T, D>(bbox, b, j);
+        }
+        for (uint32_t r = roff[u]; r < roff[u + 1]; ++r) {
+            const int2 cs = rcells[r];
+            const int32_t e1 = cbox[cs.y];
+            for (int32_t b2 = max(cbox[cs.x], b + 1); b2 < e1; ++b2) {
+                const int32_t s2 = rsnap[b2];
+                if (s2 < 0 || s2 == sb) continue;
+                double gap2 = 0.0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    const double g = fmax(0.0, fmax(box_lo<T, D>(bbox, b2, j) - hi[j],
+                                                    lo[j] - box_hi<T, D>(bbox, b2, j)));
+                    gap2 += g * g;
+                }
+                if (gap2 > gap2_max) continue;
+                if (list) list[k] = b2;
+                ++k;
+            }
+        }
+    }
+    if (!list) {
+        if (b < nbox) cnt[b] = k;
+        // 64-bit total, so that the caller can check it before a 32-bit scan
+        unsigned long long w = k;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+        if (lane_id() == 0 && w) atomicAdd(total, w);
+    }
+}
+
+// Union over the open pairs: one wave per 64-slot chunk of a box b; the
+// chunk's core rows (lanes) meet the boxes of b's open list one by one.  A
+// pair is skipped when the two boxes are joined by now (live roots) or no
+// core of the chunk is within eps of the other box's bounding box;
+// otherwise the other box's cores stream through the wave until the first
+// within-eps pair, which joins the boxes.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_union_list(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const OrdKey<T>* __restrict__ bbox, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
+    const uint32_t* __restrict__ off, const int32_t* __restrict__ list, int32_t* bparent) {
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
+    if (wid >= nitems) return;
+    const int2 it = bitems[wid];
+    const int32_t b = uniform(it.x);
+    const uint32_t l0 = uniform(static_cast<int32_t>(off[b]));
+    const uint32_t l1 = uniform(static_cast<int32_t>(off[b + 1]));
+    if (l0 == l1) return;
+    const int32_t end = uniform(bstart[b + 1]);
+    const int32_t p = it.y + lane_id();
+    const bool pcore = p < end && core[p];
+    if (!__any(pcore)) return;
+    const JobParams<T>& jp = jobs[uniform(job_of_cell[uniform(bcell[b])])];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    T pc[D];
+    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+    int32_t rb = uniform(uf_find(bparent, b));
+    for (uint32_t l = l0; l < l1; ++l) {
+        const int32_t b2 = uniform(list[l]);
+        double gmin2, gmax2;
+        point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+        const bool act = pcore && gmin2 <= gap2_max;
+        if (!__any(act)) continue;
+        const int32_t rb2 = uniform(uf_find(bparent, b2));
+        rb = uniform(uf_find(bparent, rb));
+        if (rb2 == rb) continue;
+        bool hit = false;
+        auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+            if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+        };
+        auto stop = [&] { return static_cast<bool>(__any(hit)); };
+        if (visit_range<T, D>(soa, ld, core, nullptr, uniform(bstart[b2]),
+                              uniform(bstart[b2 + 1]), body, stop)) {
+            if (lane_id() == 0) uf_unite(bparent, rb, rb2);
+            rb = uniform(uf_find(bparent, b));
         }
     }
 }
@@ -535,7 +645,7 @@ void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
     hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
-                       a.job_of_cell, a.jobs, bparent);
+                       a.job_of_cell, a.jobs, a.rcenter, bparent);
     hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        a.nbox, bmincore, bparent, rsnap);
     HPD_HIP(hipGetLastError());
@@ -550,6 +660,28 @@ void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
                        a.job_of_cell, a.jobs, rsnap, bparent);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* rsnap, uint32_t* cnt, int32_t* list,
+                     unsigned long long* total, hipStream_t s) {
+    using U = OrdKey<T>;
+    hipLaunchKernelGGL((k_box_scan<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
+                       s, a.nbox, a.bcell, a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox),
+                       a.job_of_cell, a.jobs, rsnap, cnt, list, total);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint32_t* off,
+                           const int32_t* list, int32_t* bparent, hipStream_t s) {
+    using U = OrdKey<T>;
+    hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
+                       reinterpret_cast<const U*>(a.bbox), core, a.job_of_cell, a.jobs, off, list,
+                       bparent);
     HPD_HIP(hipGetLastError());
 }
 
@@ -584,7 +716,12 @@ void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent
                                         hipStream_t);                                             \
     template void launch_box_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, const uint8_t*, const int32_t*, int32_t*, \
-                                        hipStream_t);
+                                        hipStream_t);                                             \
+    template void launch_box_scan<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
+                                        const int32_t*, uint32_t*, int32_t*,                      \
+                                        unsigned long long*, hipStream_t);                        \
+    template void launch_box_union_list<T, D>(const BoxArgs<T>&, const uint8_t*, const uint32_t*, \
+                                              const int32_t*, int32_t*, hipStream_t);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \
     HPD_INST_BOX_KEYS(T, 2)   \

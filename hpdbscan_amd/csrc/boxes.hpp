@@ -27,6 +27,7 @@ struct BoxArgs {
     void* bbox = nullptr;              // ordered-key min/max per box and dim
     const JobParams<T>* jobs = nullptr;
     const int32_t* job_of_cell = nullptr;
+    const int32_t* rcenter = nullptr;  // index of every cell's own neighbour row
 };
 
 // key = (occupied-cell index << sub_bits) | sub-box digit, val = position
@@ -81,6 +82,20 @@ template <typename T, int D>
 void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
                      int32_t* bparent, hipStream_t s);
+// The pairs the hook forest leaves open.  launch_box_scan with list ==
+// nullptr writes cnt[b] (nbox entries) and adds their sum to *total (64-bit,
+// zeroed by the caller); with a list it writes every box's open partners to
+// list[cnt[b] ..] (cnt scanned to offsets in between).
+// launch_box_union_list joins the listed pairs that hold a within-eps core
+// pair; off has nbox + 1 entries.  When the list does not fit, the caller
+// runs launch_box_join instead.
+template <typename T, int D>
+void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                     const int32_t* rsnap, uint32_t* cnt, int32_t* list,
+                     unsigned long long* total, hipStream_t s);
+template <typename T, int D>
+void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint32_t* off,
+                           const int32_t* list, int32_t* bparent, hipStream_t s);
 void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s);
 

@@ -348,10 +348,12 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int32_t *bmincore = nullptr, *bparent = nullptr, *compmin = nullptr, *bcluster = nullptr;
     uint8_t* btight = nullptr;
     void* bbox = nullptr;
+    uint64_t* box_sort_keys = nullptr;  // n words, unused after the box sort
     if constexpr (kFloat) {
         const int sub_bits = box_sub_bits(D);
         const int bits2 = bits_for(static_cast<uint64_t>(m - 1)) + sub_bits;
         uint64_t* k2a = A.take_n<uint64_t>(nn);
+        box_sort_keys = k2a;
         uint64_t* k2b = A.take_n<uint64_t>(nn);
         uint32_t* v2a = A.take_n<uint32_t>(nn);
         uint32_t* v2b = A.take_n<uint32_t>(nn);
@@ -460,6 +462,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         ba.bbox = bbox;
         ba.jobs = jobs;
         ba.job_of_cell = job_of_cell;
+        ba.rcenter = rcenter;
         launch_box_reduce<T, D>(ba, nullptr, nullptr, s);
         launch_box_tight<T, D>(ba, btight, &counters[5], s);
         a.box_of_slot = box_of_slot;
@@ -487,7 +490,23 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         tm.mark(M_HOOK0);
         launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, s);
         tm.mark(M_HOOK1);
-        launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
+        // the open pairs of every box, as a list in the (now unused)
+        // box-sort keys; offsets in bcnt (free after the box items)
+        int32_t* list = reinterpret_cast<int32_t*>(box_sort_keys);
+        const uint64_t cap = 2 * static_cast<uint64_t>(nn);
+        launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
+        unsigned long long* hs64 = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
+        HPD_HIP(hipMemcpyAsync(hs64 + 8, &dstats[5], sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+        HPD_HIP(hipStreamSynchronize(s));
+        ctx.stats.open_box_pairs = hs64[8];
+        if (hs64[8] <= cap) {  // then the 32-bit offsets cannot overflow
+            exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
+            launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s);
+            launch_box_union_list<T, D>(ba, core, bcnt, list, bparent, s);
+        } else {
+            launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
+        }
         tm.mark(M_JOIN1);
         launch_box_clusters(ba.nbox, bmincore, bparent, compmin, bcluster, s);
     } else {

@@ -91,6 +91,7 @@ typedef struct hpd_run_stats {
     uint64_t boxes;              /* sub-boxes of the occupied cells */
     int32_t box_union;           /* 1 if the union ran on sub-boxes */
     uint64_t neighbor_rows;      /* [first, end) slot ranges of all cells */
+    uint64_t open_box_pairs;     /* box pairs the hook forest left open */
 } hpd_run_stats;
 
 /* Thread-local message describing the last failure ("" if none). */
