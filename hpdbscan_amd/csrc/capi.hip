@@ -186,13 +186,10 @@ struct Context {
     Arena aux;
     Arena io;  // input rows and labels of the host-memory entry point
     int border_rule = HPD_BORDER_MIN_CLUSTER;
-    // cells with at most this many rows are handled one thread per row in
-    // the core count / border pass (box mode); HPD_SPARSE_CORE and
-    // HPD_SPARSE_BORDER override them.  Measured on the 10M x 3-D workload:
-    // the border pass gains 7x at 4-16, the core count loses a little at
-    // any threshold (fringe rows next to dense boxes serialise per thread).
-    int32_t sparse_core = 0;
-    int32_t sparse_border = 8;
+    // box mode: cells with at most this many rows are handled one thread
+    // per row in the core count and the border pass, the others one wave
+    // per 64-row chunk; HPD_SPARSE_MAX overrides it
+    int32_t sparse_max = 8;
     bool timing = false;
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
@@ -402,6 +399,17 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
     exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
+    // box mode: dense-cell chunks and sparse-cell rows as separate work
+    // lists; their per-cell counts live in the (now unused) run flags
+    uint32_t* dcnt = flags;
+    uint32_t* scnt = flags + nn + 1;
+    if (boxes) {
+        launch_split_counts(ustart, m, ctx.sparse_max, dcnt, scnt, s);
+        exclusive_scan_u32(dcnt, dcnt, m, tmp, &dcnt[m], s);
+        exclusive_scan_u32(scnt, scnt, m, tmp, &scnt[m], s);
+        HPD_HIP(hipMemcpyAsync(&counters[6], &dcnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        HPD_HIP(hipMemcpyAsync(&counters[7], &scnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
     HPD_HIP(hipMemcpyAsync(&counters[1], &nranges[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(&counters[2], &nchunks[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(hc, counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -409,6 +417,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     const uint32_t total_ranges = hc[1];
     const int32_t nitems = static_cast<int32_t>(hc[2]);
     const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
+    const int32_t nditems = boxes ? static_cast<int32_t>(hc[6]) : 0;
+    const int32_t nsitems = boxes ? static_cast<int32_t>(hc[7]) : 0;
     int32_t nbitems = 0;
     if (boxes) {
         launch_box_items(bstart, nbox, bcnt, tmp, &counters[4], s);
@@ -419,14 +429,19 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
 
     ctx.aux.reserve(2 * align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) +
-                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) + 1024);
+                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(nditems) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) + 2048);
     int2* ranges = ctx.aux.take_n<int2>(total_ranges);
     int2* rcells = ctx.aux.take_n<int2>(total_ranges);
     int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
     int2* bitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
+    int2* ditems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nditems)) : nullptr;
+    int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
     launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
                                rcenter, s);
     launch_work_items(ustart, m, nchunks, items, s);
+    if (boxes) launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, sitems, s);
 
     DbscanArgs<T> a;
     a.soa = soa;
@@ -471,8 +486,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         a.bstart = bstart;
         a.bcell = bcell;
         a.bbox = bbox;
-        a.sparse_core = ctx.sparse_core;
-        a.sparse_border = ctx.sparse_border;
+        a.ditems = ditems;
+        a.nditems = nditems;
+        a.sitems = sitems;
+        a.nsitems = nsitems;
     }
     tm.mark(M_INDEX);
 
@@ -777,10 +794,8 @@ int hpd_context_create(int device, hpd_context** out) {
         auto* c = new hpd_context();
         try {
             c->impl.device = device;
-            if (const char* e = std::getenv("HPD_SPARSE_CORE"))
-                c->impl.sparse_core = std::max(0, std::atoi(e));
-            if (const char* e = std::getenv("HPD_SPARSE_BORDER"))
-                c->impl.sparse_border = std::max(0, std::atoi(e));
+            if (const char* e = std::getenv("HPD_SPARSE_MAX"))
+                c->impl.sparse_max = std::max(0, std::atoi(e));
             hpd::DeviceGuard g(device);
             HPD_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
             void* hs = nullptr;

@@ -138,8 +138,7 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
                                                          uint8_t* __restrict__ core,
                                                          unsigned long long* __restrict__ ncore) {
     Chunk c;
-    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
-    if (c.end - uniform(a.ustart[c.u]) <= a.sparse_core) return;  // thread-per-row kernel
+    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c)) return;
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const uint32_t min_pts = jp.min_pts;
@@ -187,20 +186,20 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
 }
 
-// Thread-per-row core counts for the rows of sparse cells (at most
-// a.sparse_core rows): the same box-by-box count as k_core_boxes, with every
-// thread walking its own row's neighbourhood.
+// Thread-per-row core counts for the rows of sparse cells (a.sitems): the
+// same box-by-box count as k_core_boxes, with every thread walking its own
+// row's neighbourhood.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
                                                           uint8_t* __restrict__ core,
                                                           unsigned long long* __restrict__ ncore) {
-    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     bool is_core = false;
-    if (s < a.n) {
-        const int32_t p = static_cast<int32_t>(s);
+    if (i < a.nsitems) {
+        const int2 it = a.sitems[i];
+        const int32_t p = it.x, u = it.y;
         const int32_t b = a.box_of_slot[p];
-        const int32_t u = a.bcell[b];
-        if (a.ustart[u + 1] - a.ustart[u] <= a.sparse_core) {
+        {
             const JobParams<T>& jp = a.jobs[a.job_of_cell[u]];
             const uint32_t min_pts = jp.min_pts;
             uint32_t cnt = (a.btight[b] & 2) ? min_pts : 0u;
@@ -480,8 +479,7 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
                                                            unsigned long long* __restrict__ stats,
                                                            int2* __restrict__ amb_list) {
     Chunk c;
-    if (!fetch_chunk(a.items, a.nitems, a.ustart, a.roff, c)) return;
-    if (c.end - uniform(a.ustart[c.u]) <= a.sparse_border) return;  // thread-per-row kernel
+    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c)) return;
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
@@ -546,8 +544,8 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
     }
 }
 
-// Thread-per-row border labels for the non-core rows of sparse cells; same
-// box-by-box search as k_border_boxes.
+// Thread-per-row border labels for the non-core rows of sparse cells
+// (a.sitems); same box-by-box search as k_border_boxes.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> a,
                                                             const uint8_t* __restrict__ core,
@@ -558,10 +556,11 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     bool active = false, amb = false;
     int32_t best = INT_MAX, p = 0, u = 0;
-    if (s < a.n) {
-        p = static_cast<int32_t>(s);
-        u = a.bcell[a.box_of_slot[p]];
-        active = !core[p] && a.ustart[u + 1] - a.ustart[u] <= a.sparse_border;
+    if (s < a.nsitems) {
+        const int2 it = a.sitems[s];
+        p = it.x;
+        u = it.y;
+        active = !core[p];
     }
     if (active) {
         const JobParams<T>& jp = a.jobs[a.job_of_cell[u]];
@@ -755,10 +754,11 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
     if (a.nitems <= 0) return;
     if constexpr (std::is_floating_point<T>::value) {
         if (a.bbox) {
-            hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)),
-                               dim3(kThreads), 0, s, a, core, ncore);
-            if (a.sparse_core > 0)
-                hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(a.n, kThreads)),
+            if (a.nditems > 0)
+                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(a.nditems, kWavesPerBlock)),
+                                   dim3(kThreads), 0, s, a, core, ncore);
+            if (a.nsitems > 0)
+                hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, a, core, ncore);
             HPD_HIP(hipGetLastError());
             return;
@@ -813,10 +813,12 @@ void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     if constexpr (std::is_floating_point<T>::value) {
         if (bcluster && a.bbox) {
-            hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core,
-                               bcluster, a.inrow, labels, stats, amb_list);
-            if (a.sparse_border > 0)
-                hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.n, kThreads)),
+            if (a.nditems > 0)
+                hipLaunchKernelGGL((k_border_boxes<T, D>),
+                                   dim3(blocks_for(a.nditems, kWavesPerBlock)), dim3(kThreads), 0,
+                                   s, a, core, bcluster, a.inrow, labels, stats, amb_list);
+            if (a.nsitems > 0)
+                hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, a, core, bcluster, labels, stats,
                                    amb_list);
             HPD_HIP(hipGetLastError());

@@ -36,11 +36,13 @@ struct DbscanArgs {
     const int32_t* bstart = nullptr;
     const int32_t* bcell = nullptr;         // occupied cell of every box
     const void* bbox = nullptr;
-    // box mode: rows of cells with at most this many rows are handled one
-    // thread per row instead of one wave per 64-row chunk (0: never), in
-    // the core count and in the border pass
-    int32_t sparse_core = 0;
-    int32_t sparse_border = 0;
+    // box mode: the core count and the border pass take the chunks of dense
+    // cells from ditems (one wave each) and the rows of sparse cells from
+    // sitems, (row, cell) per row (one thread each)
+    const int2* ditems = nullptr;
+    int32_t nditems = 0;
+    const int2* sitems = nullptr;
+    int32_t nsitems = 0;
 };
 
 // core flag of every slot; with box tables the candidates are visited box

@@ -207,7 +207,53 @@ __global__ __launch_bounds__(256) void k_work_items(const int32_t* __restrict__ 
     for (int32_t p = s; p < e; p += kWave) items[w++] = make_int2(u, p);
 }
 
+// Per cell: 64-row chunks if the cell has more than `sparse_max` rows
+// (dense), else its rows one by one (sparse).
+__global__ __launch_bounds__(256) void k_split_counts(const int32_t* __restrict__ ustart, int32_t m,
+                                                      int32_t sparse_max,
+                                                      uint32_t* __restrict__ dcnt,
+                                                      uint32_t* __restrict__ scnt) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const int32_t c = ustart[u + 1] - ustart[u];
+    const bool sparse = c <= sparse_max;
+    dcnt[u] = sparse ? 0u : static_cast<uint32_t>((c + kWave - 1) / kWave);
+    scnt[u] = sparse ? static_cast<uint32_t>(c) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_split_items(const int32_t* __restrict__ ustart, int32_t m,
+                                                     int32_t sparse_max,
+                                                     const uint32_t* __restrict__ doff,
+                                                     const uint32_t* __restrict__ soff,
+                                                     int2* __restrict__ ditems,
+                                                     int2* __restrict__ sitems) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const int32_t s = ustart[u], e = ustart[u + 1];
+    if (e - s <= sparse_max) {
+        uint32_t w = soff[u];
+        for (int32_t p = s; p < e; ++p) sitems[w++] = make_int2(p, u);
+    } else {
+        uint32_t w = doff[u];
+        for (int32_t p = s; p < e; p += kWave) ditems[w++] = make_int2(u, p);
+    }
+}
+
 }  // namespace
+
+void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, uint32_t* dcnt,
+                         uint32_t* scnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_split_counts, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+                       sparse_max, dcnt, scnt);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
+                        const uint32_t* soff, int2* ditems, int2* sitems, hipStream_t s) {
+    hipLaunchKernelGGL(k_split_items, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+                       sparse_max, doff, soff, ditems, sitems);
+    HPD_HIP(hipGetLastError());
+}
 
 template <typename K>
 void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,

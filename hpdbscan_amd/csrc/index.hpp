@@ -32,6 +32,15 @@ void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m,
                           hipStream_t s);
 
 void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s);
+
+// Work split of the neighbour kernels: cells with more than sparse_max rows
+// give one item (cell, first row) per 64-row chunk (dcnt: chunks per cell),
+// the others one item (row, cell) per row (scnt: rows per cell).  doff /
+// soff are the exclusive scans of dcnt / scnt.
+void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, uint32_t* dcnt,
+                         uint32_t* scnt, hipStream_t s);
+void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
+                        const uint32_t* soff, int2* ditems, int2* sitems, hipStream_t s);
 void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,
                        hipStream_t s);
 
