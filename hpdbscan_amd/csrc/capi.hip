@@ -190,6 +190,7 @@ struct Context {
     // per row in the core count and the border pass, the others one wave
     // per 64-row chunk; HPD_SPARSE_MAX overrides it
     int32_t sparse_max = 8;
+    bool sparse_core = false;  // HPD_SPARSE_CORE=1: the core count uses the split too
     bool timing = false;
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
@@ -486,6 +487,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         a.bstart = bstart;
         a.bcell = bcell;
         a.bbox = bbox;
+        a.sparse_core = ctx.sparse_core;
         a.ditems = ditems;
         a.nditems = nditems;
         a.sitems = sitems;
@@ -796,6 +798,7 @@ int hpd_context_create(int device, hpd_context** out) {
             c->impl.device = device;
             if (const char* e = std::getenv("HPD_SPARSE_MAX"))
                 c->impl.sparse_max = std::max(0, std::atoi(e));
+            if (const char* e = std::getenv("HPD_SPARSE_CORE")) c->impl.sparse_core = std::atoi(e) != 0;
             hpd::DeviceGuard g(device);
             HPD_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
             void* hs = nullptr;

@@ -70,6 +70,94 @@ __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int3
     return true;
 }
 
+// Wave-uniform copy of lane j's value (j wave-uniform): v_readlane.
+__device__ __forceinline__ int32_t bcast(int32_t v, int j) {
+    return __builtin_amdgcn_readlane(v, j);
+}
+__device__ __forceinline__ double bcast(double v, int j) {
+    const long long x = __double_as_longlong(v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), j));
+    const uint32_t hi =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), j));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// Squared distances from a row to the box [lo, hi] (double): gmin2 to its
+// nearest point, gmax2 to its farthest corner.
+template <typename T, int D>
+__device__ __forceinline__ void point_box_lh(const double* lo, const double* hi, const T* pc,
+                                             double& gmin2, double& gmax2) {
+    gmin2 = 0.0;
+    gmax2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double v = static_cast<double>(pc[j]);
+        const double g = fmax(0.0, fmax(lo[j] - v, v - hi[j]));
+        const double f = fmax(v - lo[j], hi[j] - v);
+        gmin2 += g * g;
+        gmax2 += f * f;
+    }
+}
+
+// Visit the boxes of chunk c's neighbour rows, the cell's own row first.
+// The box table is fetched up to 64 boxes at a time, one box per lane (the
+// rows' box ranges likewise, one row per lane), and handed to fn box by box
+// with readlane, so a box costs no dependent memory access of its own.
+// fn(b2, lo, hi, q0, q1, tag) gets the box, its bounding box (double), its
+// rows [q0, q1) and tag[b2] (0 without tag), all wave-uniform; it returns
+// true to stop.
+template <typename T, int D, typename Fn>
+__device__ __forceinline__ void for_each_nbr_box(const DbscanArgs<T>& a, const Chunk& c,
+                                                 const int32_t* __restrict__ tag, Fn&& fn) {
+    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
+    const int32_t nrows = c.r1 - c.r0;
+    const int32_t center = uniform(a.rcenter[c.u]);
+    const int lane = lane_id();
+    for (int32_t g0 = 0; g0 < nrows; g0 += kWave) {
+        const int32_t i = g0 + lane;  // visiting order: own row, then the others
+        int32_t rb0 = 0, rcnt = 0;
+        if (i < nrows) {
+            const int32_t r = i == 0 ? center : (c.r0 + i - 1 < center ? c.r0 + i - 1 : c.r0 + i);
+            const int2 cs = a.rcells[r];
+            rb0 = a.cbox[cs.x];
+            rcnt = a.cbox[cs.y] - rb0;
+        }
+        int32_t incl = rcnt;  // inclusive prefix over the lanes
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int32_t t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        const int32_t excl = incl - rcnt;
+        const int32_t total = bcast(incl, kWave - 1);
+        const int32_t grows = min(kWave, nrows - g0);
+        for (int32_t k0 = 0; k0 < total; k0 += kWave) {
+            const int32_t kk = min(k0 + lane, total - 1);
+            int32_t row = 0;  // last row whose first box index is <= kk
+            for (int32_t t = 1; t < grows; ++t) row = bcast(excl, t) <= kk ? t : row;
+            const int32_t bl = __shfl(rb0, row, kWave) + (kk - __shfl(excl, row, kWave));
+            double lol[D], hil[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                lol[j] = box_lo<T, D>(bbox, bl, j);
+                hil[j] = box_hi<T, D>(bbox, bl, j);
+            }
+            const int32_t q0l = a.bstart[bl], q1l = a.bstart[bl + 1];
+            const int32_t tagl = tag ? tag[bl] : 0;
+            const int32_t nk = min(kWave, total - k0);
+            for (int32_t j = 0; j < nk; ++j) {
+                double lo[D], hi[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    lo[d] = bcast(lol[d], j);
+                    hi[d] = bcast(hil[d], j);
+                }
+                if (fn(bcast(bl, j), lo, hi, bcast(q0l, j), bcast(q1l, j), bcast(tagl, j))) return;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ int32_t wave_min(int32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -122,8 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
-    const unsigned long long nc = __popcll(ballot(is_core));
-    if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
+    (void)ncore;
 }
 
 // Box-mode core counts.  For every neighbour box: lanes whose row is farther
@@ -144,7 +231,6 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     const uint32_t min_pts = jp.min_pts;
     const double gap2_max = jp.gap2_max;
     const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
-    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     T pc[D];
@@ -153,19 +239,13 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     // rows of a tight box with >= minPts rows are core without counting
     if (valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = min_pts;
     bool need = valid && cnt < min_pts;
-    const int32_t center = uniform(a.rcenter[c.u]);
-    for (int32_t i = -1; i < c.r1 - c.r0 && __any(need); ++i) {
-        const int32_t r = i < 0 ? center : c.r0 + i;
-        if (i >= 0 && r == center) continue;
-        const int2 cs = a.rcells[r];
-        const int32_t e0 = uniform(a.cbox[uniform(cs.x)]);
-        const int32_t e1 = uniform(a.cbox[uniform(cs.y)]);
-        for (int32_t b2 = e0; b2 < e1; ++b2) {
+    if (__any(need)) {
+        for_each_nbr_box<T, D>(a, c, nullptr, [&](int32_t, const double* lo, const double* hi,
+                                                  int32_t q0, int32_t q1, int32_t) {
             double gmin2, gmax2;
-            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
             const bool near = need && gmin2 <= gap2_max;
-            if (!__any(near)) continue;
-            const int32_t q0 = uniform(a.bstart[b2]), q1 = uniform(a.bstart[b2 + 1]);
+            if (!__any(near)) return false;
             const bool whole = near && gmax2 <= full2;
             if (whole) cnt += static_cast<uint32_t>(q1 - q0);
             const bool part = near && !whole;
@@ -177,13 +257,12 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
                 visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, q0, q1, body, stop);
             }
             need = valid && cnt < min_pts;
-            if (!__any(need)) break;
-        }
+            return !__any(need);
+        });
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
-    const unsigned long long nc = __popcll(ballot(is_core));
-    if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
+    (void)ncore;
 }
 
 // Thread-per-row core counts for the rows of sparse cells (a.sitems): the
@@ -234,8 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
             core[p] = is_core ? 1 : 0;
         }
     }
-    const unsigned long long nc = __popcll(ballot(is_core));
-    if (lane_id() == 0 && nc) atomicAdd(ncore, nc);
+    (void)ncore;
 }
 
 template <typename T, int D>
@@ -446,12 +524,10 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
     }
     if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
     const bool is_amb = active && best != INT_MAX && best != worst;
-    const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
     const uint64_t amb_mask = ballot(is_amb);
     const unsigned long long amb = __popcll(amb_mask);
     unsigned long long base = 0;
     if (lane_id() == 0) {
-        if (noise) atomicAdd(&stats[0], noise);
         if (amb) base = atomicAdd(&stats[1], amb);
     }
     // record (slot, cell) of every border row next to two or more clusters
@@ -485,7 +561,6 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
     const double gap2_max = jp.gap2_max;
     const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
     const int64_t first_pos = jp.row_begin;
-    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
     const int32_t p = c.first + lane_id();
     const bool valid = p < c.end;
     const bool active = valid && !core[p];
@@ -494,46 +569,36 @@ __global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a
     load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
     int32_t best = INT_MAX;
     bool amb = false;
-    const int32_t center = uniform(a.rcenter[c.u]);
-    for (int32_t i = -1; i < c.r1 - c.r0; ++i) {
-        const int32_t r = i < 0 ? center : c.r0 + i;
-        if (i >= 0 && r == center) continue;
-        const int2 cs = a.rcells[r];
-        const int32_t e0 = uniform(a.cbox[uniform(cs.x)]);
-        const int32_t e1 = uniform(a.cbox[uniform(cs.y)]);
-        for (int32_t b2 = e0; b2 < e1; ++b2) {
-            const int32_t cl = uniform(bcluster[b2]);
-            if (cl < 0) continue;  // no cores
-            const bool want = active && cl != best && (cl < best || !amb);
-            if (!__any(want)) continue;
-            double gmin2, gmax2;
-            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
-            const bool near = want && gmin2 <= gap2_max;
-            if (!__any(near)) continue;
-            bool hit = near && gmax2 <= full2;
-            const bool part = near && !hit;
-            if (__any(part)) {
-                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-                    if (part && !hit && cq && within<D>(pc, qc, eps2)) hit = true;
-                };
-                auto stop = [&] { return static_cast<bool>(__all(!part || hit)); };
-                visit_range<T, D>(a.soa, a.ld, core, nullptr, uniform(a.bstart[b2]),
-                                  uniform(a.bstart[b2 + 1]), body, stop);
-            }
-            if (hit) {
-                if (best != INT_MAX) amb = true;
-                best = min(best, cl);
-            }
+    for_each_nbr_box<T, D>(a, c, bcluster, [&](int32_t, const double* lo, const double* hi,
+                                               int32_t q0, int32_t q1, int32_t cl) {
+        if (cl < 0) return false;  // no cores
+        const bool want = active && cl != best && (cl < best || !amb);
+        if (!__any(want)) return false;
+        double gmin2, gmax2;
+        point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+        const bool near = want && gmin2 <= gap2_max;
+        if (!__any(near)) return false;
+        bool hit = near && gmax2 <= full2;
+        const bool part = near && !hit;
+        if (__any(part)) {
+            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                if (part && !hit && cq && within<D>(pc, qc, eps2)) hit = true;
+            };
+            auto stop = [&] { return static_cast<bool>(__all(!part || hit)); };
+            visit_range<T, D>(a.soa, a.ld, core, nullptr, q0, q1, body, stop);
         }
-    }
+        if (hit) {
+            if (best != INT_MAX) amb = true;
+            best = min(best, cl);
+        }
+        return false;
+    });
     if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
     const bool is_amb = active && amb;
-    const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
     const uint64_t amb_mask = ballot(is_amb);
     const unsigned long long namb = __popcll(amb_mask);
     unsigned long long base = 0;
     if (lane_id() == 0) {
-        if (noise) atomicAdd(&stats[0], noise);
         if (namb) base = atomicAdd(&stats[1], namb);
     }
     if (amb_list && namb) {
@@ -597,12 +662,10 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
             best == INT_MAX ? 0 : static_cast<int64_t>(best) - jp.row_begin + 1;
     }
     const bool is_amb = active && amb;
-    const unsigned long long noise = __popcll(ballot(active && best == INT_MAX));
     const uint64_t amb_mask = ballot(is_amb);
     const unsigned long long namb = __popcll(amb_mask);
     unsigned long long base = 0;
     if (lane_id() == 0) {
-        if (noise) atomicAdd(&stats[0], noise);
         if (namb) base = atomicAdd(&stats[1], namb);
     }
     if (amb_list && namb) {
@@ -731,6 +794,46 @@ __global__ __launch_bounds__(kThreads) void k_ot_init(const uint8_t* __restrict_
     if (i < n) f[i] = core[i] ? slot_pos(pos_of_slot, static_cast<int32_t>(i)) : INT_MAX;
 }
 
+// Number of zero labels among labels[0, n): one atomic per block.
+__global__ __launch_bounds__(kThreads) void k_count_zero(const int64_t* __restrict__ v, int64_t n,
+                                                         unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[kWavesPerBlock];
+    unsigned long long c = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kThreads)
+        c += v[i] == 0 ? 1ull : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) t += part[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+// Sum of n byte flags (0/1): one atomic per block.
+__global__ __launch_bounds__(kThreads) void k_count_flags(const uint8_t* __restrict__ f, int64_t n,
+                                                          unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[kWavesPerBlock];
+    unsigned long long c = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kThreads)
+        c += f[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) t += part[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_iota(int32_t* __restrict__ v, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i < n) v[i] = static_cast<int32_t>(i);
@@ -749,24 +852,49 @@ inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n +
 
 }  // namespace
 
+// *out += number of nonzero flags among f[0, n)
+void count_flags(const uint8_t* f, int64_t n, unsigned long long* out, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t want = (n + kThreads - 1) / kThreads;
+    const unsigned grid = static_cast<unsigned>(want < 2048 ? want : 2048);
+    hipLaunchKernelGGL(k_count_flags, dim3(grid), dim3(kThreads), 0, s, f, n, out);
+    HPD_HIP(hipGetLastError());
+}
+
+// *out += number of zero labels (noise rows) among labels[0, n)
+void count_noise(const int64_t* labels, int64_t n, unsigned long long* out, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t want = (n + kThreads - 1) / kThreads;
+    const unsigned grid = static_cast<unsigned>(want < 2048 ? want : 2048);
+    hipLaunchKernelGGL(k_count_zero, dim3(grid), dim3(kThreads), 0, s, labels, n, out);
+    HPD_HIP(hipGetLastError());
+}
+
 template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s) {
     if (a.nitems <= 0) return;
     if constexpr (std::is_floating_point<T>::value) {
         if (a.bbox) {
-            if (a.nditems > 0)
-                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(a.nditems, kWavesPerBlock)),
-                                   dim3(kThreads), 0, s, a, core, ncore);
-            if (a.nsitems > 0)
-                hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
-                                   dim3(kThreads), 0, s, a, core, ncore);
-            HPD_HIP(hipGetLastError());
+            DbscanArgs<T> ac = a;
+            if (!a.sparse_core) {  // every cell's chunks as waves
+                ac.ditems = a.items;
+                ac.nditems = a.nitems;
+                ac.nsitems = 0;
+            }
+            if (ac.nditems > 0)
+                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(ac.nditems, kWavesPerBlock)),
+                                   dim3(kThreads), 0, s, ac, core, ncore);
+            if (ac.nsitems > 0)
+                hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(ac.nsitems, kThreads)),
+                                   dim3(kThreads), 0, s, ac, core, ncore);
+            count_flags(core, a.n, ncore, s);
             return;
         }
     }
     hipLaunchKernelGGL((k_core<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)), dim3(kThreads),
                        0, s, a, core, ncore);
     HPD_HIP(hipGetLastError());
+    count_flags(core, a.n, ncore, s);
 }
 
 template <typename T, int D>
@@ -822,12 +950,14 @@ void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
                                    dim3(kThreads), 0, s, a, core, bcluster, labels, stats,
                                    amb_list);
             HPD_HIP(hipGetLastError());
+            count_noise(labels, a.n, &stats[0], s);
             return;
         }
     }
     hipLaunchKernelGGL((k_border<T, D>), dim3(gw), dim3(kThreads), 0, s, a, core, rootslot, cmin,
                        cmax, a.inrow, labels, stats, amb_list);
     HPD_HIP(hipGetLastError());
+    count_noise(labels, a.n, &stats[0], s);
 }
 
 template <typename T, int D>

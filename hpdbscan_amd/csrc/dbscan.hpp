@@ -43,6 +43,7 @@ struct DbscanArgs {
     int32_t nditems = 0;
     const int2* sitems = nullptr;
     int32_t nsitems = 0;
+    bool sparse_core = false;  // core count split too (else one wave per chunk of every cell)
 };
 
 // core flag of every slot; with box tables the candidates are visited box
