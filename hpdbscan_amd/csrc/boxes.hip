@@ -372,24 +372,46 @@ __global__ __launch_bounds__(kThreads) void k_box_union_list(
     T pc[D];
     load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
     int32_t rb = uniform(uf_find(bparent, b));
-    for (uint32_t l = l0; l < l1; ++l) {
-        const int32_t b2 = uniform(list[l]);
-        double gmin2, gmax2;
-        point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
-        const bool act = pcore && gmin2 <= gap2_max;
-        if (!__any(act)) continue;
-        const int32_t rb2 = uniform(uf_find(bparent, b2));
-        rb = uniform(uf_find(bparent, rb));
-        if (rb2 == rb) continue;
-        bool hit = false;
-        auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-            if (cq && act && within<D>(pc, qc, eps2)) hit = true;
-        };
-        auto stop = [&] { return static_cast<bool>(__any(hit)); };
-        if (visit_range<T, D>(soa, ld, core, nullptr, uniform(bstart[b2]),
-                              uniform(bstart[b2 + 1]), body, stop)) {
-            if (lane_id() == 0) uf_unite(bparent, rb, rb2);
-            rb = uniform(uf_find(bparent, b));
+    for (uint32_t L0 = l0; L0 < l1; L0 += kWave) {
+        // lanes fetch 64 partners at once: box, bounding box, rows, and a
+        // root (roots only merge, so an equal root is final)
+        const uint32_t li = min(L0 + static_cast<uint32_t>(lane_id()), l1 - 1);
+        const int32_t b2l = list[li];
+        double lol[D], hil[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            lol[j] = box_lo<T, D>(bbox, b2l, j);
+            hil[j] = box_hi<T, D>(bbox, b2l, j);
+        }
+        const int32_t q0l = bstart[b2l], q1l = bstart[b2l + 1];
+        const int32_t r2l = uf_find(bparent, b2l);
+        const int32_t nk = static_cast<int32_t>(min(static_cast<uint32_t>(kWave), l1 - L0));
+        for (int32_t j = 0; j < nk; ++j) {
+            double lo[D], hi[D];
+#pragma unroll
+            for (int e = 0; e < D; ++e) {
+                lo[e] = bcast(lol[e], j);
+                hi[e] = bcast(hil[e], j);
+            }
+            double gmin2, gmax2;
+            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+            const bool act = pcore && gmin2 <= gap2_max;
+            if (!__any(act)) continue;
+            int32_t rb2 = bcast(r2l, j);
+            if (rb2 == rb) continue;
+            rb = uniform(uf_find(bparent, rb));
+            rb2 = uniform(uf_find(bparent, rb2));
+            if (rb2 == rb) continue;
+            bool hit = false;
+            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+            };
+            auto stop = [&] { return static_cast<bool>(__any(hit)); };
+            if (visit_range<T, D>(soa, ld, core, nullptr, bcast(q0l, j), bcast(q1l, j), body,
+                                  stop)) {
+                if (lane_id() == 0) uf_unite(bparent, rb, rb2);
+                rb = uniform(uf_find(bparent, b));
+            }
         }
     }
 }

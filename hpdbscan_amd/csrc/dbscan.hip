@@ -70,92 +70,12 @@ __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int3
     return true;
 }
 
-// Wave-uniform copy of lane j's value (j wave-uniform): v_readlane.
-__device__ __forceinline__ int32_t bcast(int32_t v, int j) {
-    return __builtin_amdgcn_readlane(v, j);
-}
-__device__ __forceinline__ double bcast(double v, int j) {
-    const long long x = __double_as_longlong(v);
-    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), j));
-    const uint32_t hi =
-        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), j));
-    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
-}
-
-// Squared distances from a row to the box [lo, hi] (double): gmin2 to its
-// nearest point, gmax2 to its farthest corner.
-template <typename T, int D>
-__device__ __forceinline__ void point_box_lh(const double* lo, const double* hi, const T* pc,
-                                             double& gmin2, double& gmax2) {
-    gmin2 = 0.0;
-    gmax2 = 0.0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        const double v = static_cast<double>(pc[j]);
-        const double g = fmax(0.0, fmax(lo[j] - v, v - hi[j]));
-        const double f = fmax(v - lo[j], hi[j] - v);
-        gmin2 += g * g;
-        gmax2 += f * f;
-    }
-}
-
-// Visit the boxes of chunk c's neighbour rows, the cell's own row first.
-// The box table is fetched up to 64 boxes at a time, one box per lane (the
-// rows' box ranges likewise, one row per lane), and handed to fn box by box
-// with readlane, so a box costs no dependent memory access of its own.
-// fn(b2, lo, hi, q0, q1, tag) gets the box, its bounding box (double), its
-// rows [q0, q1) and tag[b2] (0 without tag), all wave-uniform; it returns
-// true to stop.
+// The boxes of chunk c's neighbour rows (for_each_row_box).
 template <typename T, int D, typename Fn>
 __device__ __forceinline__ void for_each_nbr_box(const DbscanArgs<T>& a, const Chunk& c,
                                                  const int32_t* __restrict__ tag, Fn&& fn) {
-    const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
-    const int32_t nrows = c.r1 - c.r0;
-    const int32_t center = uniform(a.rcenter[c.u]);
-    const int lane = lane_id();
-    for (int32_t g0 = 0; g0 < nrows; g0 += kWave) {
-        const int32_t i = g0 + lane;  // visiting order: own row, then the others
-        int32_t rb0 = 0, rcnt = 0;
-        if (i < nrows) {
-            const int32_t r = i == 0 ? center : (c.r0 + i - 1 < center ? c.r0 + i - 1 : c.r0 + i);
-            const int2 cs = a.rcells[r];
-            rb0 = a.cbox[cs.x];
-            rcnt = a.cbox[cs.y] - rb0;
-        }
-        int32_t incl = rcnt;  // inclusive prefix over the lanes
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int32_t t = __shfl_up(incl, o, kWave);
-            if (lane >= o) incl += t;
-        }
-        const int32_t excl = incl - rcnt;
-        const int32_t total = bcast(incl, kWave - 1);
-        const int32_t grows = min(kWave, nrows - g0);
-        for (int32_t k0 = 0; k0 < total; k0 += kWave) {
-            const int32_t kk = min(k0 + lane, total - 1);
-            int32_t row = 0;  // last row whose first box index is <= kk
-            for (int32_t t = 1; t < grows; ++t) row = bcast(excl, t) <= kk ? t : row;
-            const int32_t bl = __shfl(rb0, row, kWave) + (kk - __shfl(excl, row, kWave));
-            double lol[D], hil[D];
-#pragma unroll
-            for (int j = 0; j < D; ++j) {
-                lol[j] = box_lo<T, D>(bbox, bl, j);
-                hil[j] = box_hi<T, D>(bbox, bl, j);
-            }
-            const int32_t q0l = a.bstart[bl], q1l = a.bstart[bl + 1];
-            const int32_t tagl = tag ? tag[bl] : 0;
-            const int32_t nk = min(kWave, total - k0);
-            for (int32_t j = 0; j < nk; ++j) {
-                double lo[D], hi[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    lo[d] = bcast(lol[d], j);
-                    hi[d] = bcast(hil[d], j);
-                }
-                if (fn(bcast(bl, j), lo, hi, bcast(q0l, j), bcast(q1l, j), bcast(tagl, j))) return;
-            }
-        }
-    }
+    for_each_row_box<T, D>(c.r0, c.r1, uniform(a.rcenter[c.u]), a.rcells, a.cbox, a.bstart,
+                           static_cast<const OrdKey<T>*>(a.bbox), tag, fn);
 }
 
 __device__ __forceinline__ int32_t wave_min(int32_t v) {

@@ -117,4 +117,94 @@ __device__ __forceinline__ void point_box(const OrdKey<T>* __restrict__ bbox, in
     }
 }
 
+// Wave-uniform copy of lane j's value (j wave-uniform): v_readlane.
+__device__ __forceinline__ int32_t bcast(int32_t v, int j) {
+    return __builtin_amdgcn_readlane(v, j);
+}
+__device__ __forceinline__ double bcast(double v, int j) {
+    const long long x = __double_as_longlong(v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), j));
+    const uint32_t hi =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), j));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// Squared distances from a row to the box [lo, hi] (double): gmin2 to its
+// nearest point, gmax2 to its farthest corner.
+template <typename T, int D>
+__device__ __forceinline__ void point_box_lh(const double* lo, const double* hi, const T* pc,
+                                             double& gmin2, double& gmax2) {
+    gmin2 = 0.0;
+    gmax2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double v = static_cast<double>(pc[j]);
+        const double g = fmax(0.0, fmax(lo[j] - v, v - hi[j]));
+        const double f = fmax(v - lo[j], hi[j] - v);
+        gmin2 += g * g;
+        gmax2 += f * f;
+    }
+}
+
+// Visit the boxes of the neighbour rows [r0, r1) of an occupied cell, the
+// cell's own row (index `center`) first.  The box table is fetched up to 64
+// boxes at a time, one box per lane (the rows' box ranges likewise, one row
+// per lane), and handed to fn box by box with readlane, so a box costs no
+// dependent memory access of its own.  fn(b2, lo, hi, q0, q1, tag) gets the
+// box, its bounding box (double), its rows [q0, q1) and tag[b2] (0 without
+// tag), all wave-uniform; it returns true to stop the visit.
+template <typename T, int D, typename Fn>
+__device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t center,
+                                                 const int2* __restrict__ rcells,
+                                                 const int32_t* __restrict__ cbox,
+                                                 const int32_t* __restrict__ bstart,
+                                                 const OrdKey<T>* __restrict__ bbox,
+                                                 const int32_t* __restrict__ tag, Fn&& fn) {
+    const int32_t nrows = r1 - r0;
+    const int lane = __lane_id();
+    for (int32_t g0 = 0; g0 < nrows; g0 += 64) {
+        const int32_t i = g0 + lane;  // visiting order: own row, then the others
+        int32_t rb0 = 0, rcnt = 0;
+        if (i < nrows) {
+            const int32_t r = i == 0 ? center : (r0 + i - 1 < center ? r0 + i - 1 : r0 + i);
+            const int2 cs = rcells[r];
+            rb0 = cbox[cs.x];
+            rcnt = cbox[cs.y] - rb0;
+        }
+        int32_t incl = rcnt;  // inclusive prefix over the lanes
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        const int32_t excl = incl - rcnt;
+        const int32_t total = bcast(incl, 63);
+        const int32_t grows = min(64, nrows - g0);
+        for (int32_t k0 = 0; k0 < total; k0 += 64) {
+            const int32_t kk = min(k0 + lane, total - 1);
+            int32_t row = 0;  // last row whose first box index is <= kk
+            for (int32_t t = 1; t < grows; ++t) row = bcast(excl, t) <= kk ? t : row;
+            const int32_t bl = __shfl(rb0, row, 64) + (kk - __shfl(excl, row, 64));
+            double lol[D], hil[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                lol[j] = box_lo<T, D>(bbox, bl, j);
+                hil[j] = box_hi<T, D>(bbox, bl, j);
+            }
+            const int32_t q0l = bstart[bl], q1l = bstart[bl + 1];
+            const int32_t tagl = tag ? tag[bl] : 0;
+            const int32_t nk = min(64, total - k0);
+            for (int32_t j = 0; j < nk; ++j) {
+                double lo[D], hi[D];
+#pragma unroll
+                for (int e = 0; e < D; ++e) {
+                    lo[e] = bcast(lol[e], j);
+                    hi[e] = bcast(hil[e], j);
+                }
+                if (fn(bcast(bl, j), lo, hi, bcast(q0l, j), bcast(q1l, j), bcast(tagl, j))) return;
+            }
+        }
+    }
+}
+
 }  // namespace hpd
