@@ -39,6 +39,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
+// rounds of root joins after the hook forest: one suffices in 1-3 dims;
+// with the finer 4+-dim boxes (k = 3) a second one still halves the open pairs
+template <int D>
+constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
@@ -260,33 +264,78 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
     const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
     // the cell's own row first: the box's nearest neighbours are there, so
-    // the first hit comes soonest
-    const int32_t center = uniform(rcenter[u]);
-    for (int32_t i = -1; i < r1 - r0; ++i) {
-        const int32_t r = i < 0 ? center : r0 + i;
-        if (i >= 0 && r == center) continue;
-        const int2 cs = rcells[r];
-        const int32_t e0 = uniform(cbox[uniform(cs.x)]);
-        if (e0 >= b) continue;  // only boxes before b
-        const int32_t e1 = uniform(min(cbox[uniform(cs.y)], b));
-        for (int32_t b2 = e0; b2 < e1; ++b2) {
-            if (bmincore[b2] == INT_MAX) continue;
+    // the first hit comes soonest; boxes before b only
+    for_each_row_box<T, D>(
+        r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, bmincore,
+        [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
+            int32_t mincore) {
+            if (mincore == INT_MAX) return false;  // no cores
             double gmin2, gmax2;
-            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
             const bool act = pcore && gmin2 <= gap2_max;
-            if (!__any(act)) continue;
+            if (!__any(act)) return false;
             bool hit = false;
             auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
                 if (cq && act && within<D>(pc, qc, eps2)) hit = true;
             };
             auto stop = [&] { return static_cast<bool>(__any(hit)); };
-            if (visit_range<T, D>(soa, ld, core, nullptr, uniform(bstart[b2]),
-                                  uniform(bstart[b2 + 1]), body, stop)) {
-                if (lane_id() == 0) bparent[b] = b2;
-                return;
-            }
-        }
-    }
+            if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+            if (lane_id() == 0) bparent[b] = b2;
+            return true;
+        },
+        b);
+}
+
+// Root join: every root of the hook forest (a box without an earlier
+// within-eps core pair) looks through its whole neighbourhood for the first
+// box of another tree holding a within-eps core pair and joins the two
+// trees (union-find, smaller root on top).  One round merges most trees of
+// a cluster, so far fewer pairs stay open for the union.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_root_join(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
+    const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
+    const int32_t* __restrict__ rsnap, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
+    const int32_t* __restrict__ rcenter, int32_t* bparent) {
+    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                                static_cast<int32_t>(threadIdx.x >> 6));
+    if (wid >= nitems) return;
+    const int2 it = bitems[wid];
+    const int32_t b = uniform(it.x);
+    if (rsnap[b] != b) return;  // not a root (or no cores)
+    const int32_t end = uniform(bstart[b + 1]);
+    const int32_t p = it.y + lane_id();
+    const bool pcore = p < end && core[p];
+    if (!__any(pcore)) return;
+    const int32_t u = uniform(bcell[b]);
+    const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    T pc[D];
+    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+    const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
+    const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
+    for_each_row_box<T, D>(
+        r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, rsnap,
+        [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
+            int32_t s2) {
+            if (s2 < 0 || s2 == b) return false;  // no cores, or b's own tree
+            double gmin2, gmax2;
+            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+            const bool act = pcore && gmin2 <= gap2_max;
+            if (!__any(act)) return false;
+            bool hit = false;
+            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+            };
+            auto stop = [&] { return static_cast<bool>(__any(hit)); };
+            if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+            if (lane_id() == 0) uf_unite(bparent, b, b2);
+            return true;
+        });
 }
 
 // Box pairs the hook forest leaves open: one thread per box b finds every
@@ -670,6 +719,16 @@ void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
                        a.job_of_cell, a.jobs, a.rcenter, bparent);
     hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        a.nbox, bmincore, bparent, rsnap);
+    // rounds of root joins, each followed by the snapshot again
+    for (int round = 0; round < kRootJoinRounds<D>; ++round) {
+        hipLaunchKernelGGL((k_box_root_join<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+                           dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart,
+                           a.bcell, a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox),
+                           static_cast<const int32_t*>(rsnap), core, a.job_of_cell, a.jobs,
+                           a.rcenter, bparent);
+        hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                           a.nbox, bmincore, bparent, rsnap);
+    }
     HPD_HIP(hipGetLastError());
 }
 

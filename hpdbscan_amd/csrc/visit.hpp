@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <type_traits>
 
@@ -152,14 +153,16 @@ __device__ __forceinline__ void point_box_lh(const double* lo, const double* hi,
 // per lane), and handed to fn box by box with readlane, so a box costs no
 // dependent memory access of its own.  fn(b2, lo, hi, q0, q1, tag) gets the
 // box, its bounding box (double), its rows [q0, q1) and tag[b2] (0 without
-// tag), all wave-uniform; it returns true to stop the visit.
+// tag), all wave-uniform; it returns true to stop the visit.  Only boxes
+// below bmax are visited.
 template <typename T, int D, typename Fn>
 __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t center,
                                                  const int2* __restrict__ rcells,
                                                  const int32_t* __restrict__ cbox,
                                                  const int32_t* __restrict__ bstart,
                                                  const OrdKey<T>* __restrict__ bbox,
-                                                 const int32_t* __restrict__ tag, Fn&& fn) {
+                                                 const int32_t* __restrict__ tag, Fn&& fn,
+                                                 int32_t bmax = INT_MAX) {
     const int32_t nrows = r1 - r0;
     const int lane = __lane_id();
     for (int32_t g0 = 0; g0 < nrows; g0 += 64) {
@@ -169,7 +172,7 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
             const int32_t r = i == 0 ? center : (r0 + i - 1 < center ? r0 + i - 1 : r0 + i);
             const int2 cs = rcells[r];
             rb0 = cbox[cs.x];
-            rcnt = cbox[cs.y] - rb0;
+            rcnt = max(0, min(cbox[cs.y], bmax) - rb0);  // boxes below bmax only
         }
         int32_t incl = rcnt;  // inclusive prefix over the lanes
 #pragma unroll
