@@ -93,19 +93,55 @@ __global__ __launch_bounds__(kThreads) void k_box_keys(const T* __restrict__ pts
     vals[pos] = static_cast<uint32_t>(pos);
 }
 
-// rows in box order: coordinates (SoA), input row and sorted position
+// rows in box order: coordinates (SoA) and input row; with it the bounding
+// box of every box (ordered keys, pre-filled with +inf / -inf keys): a box
+// inside the wave's 64 slots is stored by its first lane, one that crosses
+// the wave's edge is merged with atomics
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_gather_boxes(const T* __restrict__ pts, int64_t n,
-                                                           const uint32_t* __restrict__ pos_of_slot,
-                                                           const uint32_t* __restrict__ inrow_of_pos,
-                                                           T* __restrict__ soa, int64_t ld,
-                                                           uint32_t* __restrict__ inrow_of_slot) {
+__global__ __launch_bounds__(kThreads) void k_gather_boxes(
+    const T* __restrict__ pts, int64_t n, const uint32_t* __restrict__ pos_of_slot,
+    const uint32_t* __restrict__ inrow_of_pos, T* __restrict__ soa, int64_t ld,
+    uint32_t* __restrict__ inrow_of_slot, const int32_t* __restrict__ box_of_slot,
+    const int32_t* __restrict__ bstart, OrdKey<T>* __restrict__ bbox) {
+    using U = OrdKey<T>;
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t row = inrow_of_pos[pos_of_slot[s]];
-    inrow_of_slot[s] = row;
+    const bool valid = s < n;
+    int32_t b = -1 - lane_id();  // distinct keys for lanes past the end
+    T x[D];
+    if (valid) {
+        const uint32_t row = inrow_of_pos[pos_of_slot[s]];
+        inrow_of_slot[s] = row;
+        b = box_of_slot[s];
 #pragma unroll
-    for (int j = 0; j < D; ++j) soa[static_cast<int64_t>(j) * ld + s] = pts[static_cast<int64_t>(row) * D + j];
+        for (int j = 0; j < D; ++j) {
+            x[j] = pts[static_cast<int64_t>(row) * D + j];
+            soa[static_cast<int64_t>(j) * ld + s] = x[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; ++j) x[j] = T(0);
+    }
+    const bool first = run_head(b);  // every lane takes part in the shuffles
+    const bool head = valid && first;
+    const int64_t w0 = s - lane_id();  // first slot of this wave
+    bool inside = false;
+    if (head) inside = bstart[b] >= w0 && bstart[b + 1] <= w0 + kWave;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const U k = ord_key(x[j]);
+        const U lo = run_suffix_min<U>(k, b);
+        const U hi = run_suffix_max<U>(k, b);
+        if (head) {
+            U* dst = &bbox[(static_cast<int64_t>(b) * D + j) * 2];
+            if (inside) {
+                dst[0] = lo;
+                dst[1] = hi;
+            } else {
+                atomicMin(&dst[0], lo);
+                atomicMax(&dst[1], hi);
+            }
+        }
+    }
 }
 
 template <typename K>
@@ -123,7 +159,9 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
                                                         int sub_bits, int32_t* __restrict__ bstart,
                                                         int32_t* __restrict__ bcell,
                                                         int32_t* __restrict__ box_of_slot,
-                                                        const uint32_t* __restrict__ nbox_dev) {
+                                                        const uint32_t* __restrict__ nbox_dev,
+                                                        void* __restrict__ bbox, int bbox_keys,
+                                                        int key_bytes) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i < n) {
         const uint32_t b = scan[i] + flags[i] - 1u;
@@ -131,6 +169,16 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
         if (flags[i]) {
             bstart[b] = static_cast<int32_t>(i);
             bcell[b] = static_cast<int32_t>(static_cast<uint64_t>(keys[i]) >> sub_bits);
+            // empty bounding box: min keys all ones, max keys zero
+            if (bbox) {
+                const int64_t k0 = static_cast<int64_t>(b) * bbox_keys;
+                for (int k = 0; k < bbox_keys; ++k) {
+                    if (key_bytes == 8)
+                        static_cast<unsigned long long*>(bbox)[k0 + k] = (k & 1) ? 0ull : ~0ull;
+                    else
+                        static_cast<uint32_t*>(bbox)[k0 + k] = (k & 1) ? 0u : ~0u;
+                }
+            }
         }
     }
     if (i == 0) bstart[*nbox_dev] = static_cast<int32_t>(n);
@@ -162,46 +210,6 @@ __device__ __forceinline__ U wave_max_u(U v) {
         v = w > v ? w : v;
     }
     return v;
-}
-
-// Per box: bounding box (as ordered keys) and the smallest position of a
-// core member.  One wave per 64-row chunk of a box.
-template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_box_reduce(const T* __restrict__ soa, int64_t ld,
-                                                         const int2* __restrict__ bitems,
-                                                         int32_t nitems,
-                                                         const int32_t* __restrict__ bstart,
-                                                         const uint32_t* __restrict__ pos_of_slot,
-                                                         const uint8_t* __restrict__ core,
-                                                         OrdKey<T>* __restrict__ bbox,
-                                                         int32_t* __restrict__ bmincore) {
-    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                                static_cast<int32_t>(threadIdx.x >> 6));
-    if (wid >= nitems) return;
-    const int2 it = bitems[wid];
-    const int32_t b = uniform(it.x);
-    const int32_t end = uniform(bstart[b + 1]);
-    const int32_t s = it.y + __lane_id();
-    const bool valid = s < end;
-    using U = OrdKey<T>;
-    if (!core) {
-#pragma unroll
-        for (int j = 0; j < D; ++j) {
-            const U kv = valid ? ord_key(soa[static_cast<int64_t>(j) * ld + s]) : U(0);
-            const U lo = wave_min_u<U>(valid ? kv : ~U(0));
-            const U hi = wave_max_u<U>(valid ? kv : U(0));
-            if (__lane_id() == 0) {
-                atomicMin(&bbox[(static_cast<int64_t>(b) * D + j) * 2], lo);
-                atomicMax(&bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1], hi);
-            }
-        }
-    } else {
-        const int32_t pc = (valid && core[s]) ? static_cast<int32_t>(pos_of_slot[s]) : INT_MAX;
-        int32_t m = pc;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
-        if (__lane_id() == 0 && m != INT_MAX) atomicMin(&bmincore[b], m);
-    }
 }
 
 // tight flag per box: dist2(extent) <= EPS2 in the reference's order
@@ -596,11 +604,6 @@ __global__ __launch_bounds__(kThreads) void k_box_items_fill(const int32_t* __re
     for (int32_t s = bstart[b]; s < bstart[b + 1]; s += kWave) items[w++] = make_int2(b, s);
 }
 
-template <typename U>
-__global__ __launch_bounds__(kThreads) void k_fill_bbox(U* __restrict__ bbox, int64_t n) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) bbox[i] = (i & 1) ? U(0) : ~U(0);
-}
 
 __global__ __launch_bounds__(kThreads) void k_fill_i32(int32_t* __restrict__ a, int32_t v,
                                                        int64_t n) {
@@ -638,22 +641,26 @@ void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, cons
 template <typename T, int D>
 void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
                          const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
+                         const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
                          hipStream_t s) {
+    using U = OrdKey<T>;
     hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                       pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot);
+                       pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot, box_of_slot,
+                       bstart, static_cast<U*>(bbox));
     HPD_HIP(hipGetLastError());
 }
 
 template <typename K>
 void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
                       int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
-                      hipStream_t s) {
+                      void* bbox, int bbox_keys, int key_bytes, hipStream_t s) {
     hipLaunchKernelGGL(k_box_flags<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
                        flags);
     exclusive_scan_u32(flags, flags + n, n, scan_tmp, nbox_dev, s);
     hipLaunchKernelGGL(k_box_table<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
                        static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(flags + n),
-                       sub_bits, bstart, bcell, box_of_slot, static_cast<const uint32_t*>(nbox_dev));
+                       sub_bits, bstart, bcell, box_of_slot, static_cast<const uint32_t*>(nbox_dev),
+                       bbox, bbox_keys, key_bytes);
     HPD_HIP(hipGetLastError());
 }
 
@@ -676,24 +683,6 @@ void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* 
                            hipStream_t s) {
     hipLaunchKernelGGL(k_box_items_fill, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
                        bstart, nbox, off, items);
-    HPD_HIP(hipGetLastError());
-}
-
-template <typename T, int D>
-void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bmincore,
-                       hipStream_t s) {
-    using U = OrdKey<T>;
-    if (!core) {
-        const int64_t nb = static_cast<int64_t>(a.nbox) * D * 2;
-        hipLaunchKernelGGL(k_fill_bbox<U>, dim3(blocks_for(nb, kThreads)), dim3(kThreads), 0, s,
-                           reinterpret_cast<U*>(a.bbox), nb);
-    } else {
-        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                           bmincore, INT_MAX, static_cast<int64_t>(a.nbox));
-    }
-    hipLaunchKernelGGL((k_box_reduce<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
-                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart,
-                       a.pos_of_slot, core, reinterpret_cast<U*>(a.bbox), bmincore);
     HPD_HIP(hipGetLastError());
 }
 
@@ -766,6 +755,12 @@ void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint3
     HPD_HIP(hipGetLastError());
 }
 
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, p, v, n);
+    HPD_HIP(hipGetLastError());
+}
+
 void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
@@ -787,9 +782,8 @@ void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent
                                                   const JobParams<T>*, int32_t, uint64_t*,        \
                                                   uint32_t*, hipStream_t);                        \
     template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, const uint32_t*,  \
-                                            T*, int64_t, uint32_t*, hipStream_t);                 \
-    template void launch_box_reduce<T, D>(const BoxArgs<T>&, const uint8_t*, int32_t*,            \
-                                          hipStream_t);                                           \
+                                            T*, int64_t, uint32_t*, const int32_t*,               \
+                                            const int32_t*, void*, hipStream_t);                  \
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
@@ -816,8 +810,10 @@ HPD_INST_BOX_ALL_D(float)
 HPD_INST_BOX_ALL_D(double)
 
 template void launch_box_table<uint32_t>(const uint32_t*, int64_t, int, uint32_t*, uint32_t*,
-                                         int32_t*, int32_t*, int32_t*, uint32_t*, hipStream_t);
+                                         int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
+                                         hipStream_t);
 template void launch_box_table<uint64_t>(const uint64_t*, int64_t, int, uint32_t*, uint32_t*,
-                                         int32_t*, int32_t*, int32_t*, uint32_t*, hipStream_t);
+                                         int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
+                                         hipStream_t);
 
 }  // namespace hpd

@@ -37,15 +37,18 @@ void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, cons
                      uint32_t* vals, hipStream_t s);
 
 template <typename T, int D>
+// also the bounding boxes (bbox, initialised by launch_box_table)
 void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
                          const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
+                         const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
                          hipStream_t s);
 
-// flags: 2n + 2 words; bstart: nbox + 1; writes *nbox_dev
+// flags: 2n + 2 words; bstart: nbox + 1; writes *nbox_dev; bbox (when not
+// null, bbox_keys keys of key_bytes each per box) is set to empty boxes
 template <typename K>
 void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
                       int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
-                      hipStream_t s);
+                      void* bbox, int bbox_keys, int key_bytes, hipStream_t s);
 
 void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
                        hipStream_t s);
@@ -55,12 +58,6 @@ void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32
                       uint32_t* total_dev, hipStream_t s);
 void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off, int2* items,
                            hipStream_t s);
-
-// core == nullptr: bounding boxes; otherwise the smallest core position of
-// every box into bmincore (INT_MAX for boxes without cores)
-template <typename T, int D>
-void launch_box_reduce(const BoxArgs<T>& a, const uint8_t* core, int32_t* bmincore,
-                       hipStream_t s);
 
 // btight[b]: bit 0 tight, bit 1 tight with >= minPts rows; *nloose counts
 // boxes of two or more rows that are not tight
@@ -96,6 +93,9 @@ void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
 template <typename T, int D>
 void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint32_t* off,
                            const int32_t* list, int32_t* bparent, hipStream_t s);
+// p[0, n) = v
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+
 void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s);
 

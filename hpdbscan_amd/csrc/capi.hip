@@ -378,7 +378,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
             slot_pos = ib ? v2b : v2a;
             sorted2 = ib ? static_cast<const void*>(kb) : static_cast<const void*>(ka);
             launch_box_table<uint32_t>(static_cast<const uint32_t*>(sorted2), n, sub_bits, flags, tmp,
-                                       bstart, bcell, box_of_slot, &counters[3], s);
+                                       bstart, bcell, box_of_slot, &counters[3], bbox, 2 * D,
+                                       static_cast<int>(sizeof(T)), s);
         } else {
             launch_box_keys<T, D, uint64_t>(pts, n, inrow_of_pos, flags, flags + n, jobs, njobs,
                                             k2a, v2a, s);
@@ -386,10 +387,12 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
             slot_pos = ib ? v2b : v2a;
             sorted2 = ib ? static_cast<const void*>(k2b) : static_cast<const void*>(k2a);
             launch_box_table<uint64_t>(static_cast<const uint64_t*>(sorted2), n, sub_bits, flags, tmp,
-                                       bstart, bcell, box_of_slot, &counters[3], s);
+                                       bstart, bcell, box_of_slot, &counters[3], bbox, 2 * D,
+                                       static_cast<int>(sizeof(T)), s);
         }
         pos_of_slot = slot_pos;
-        launch_gather_boxes<T, D>(pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot, s);
+        launch_gather_boxes<T, D>(pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot,
+                                  box_of_slot, bstart, bbox, s);
     } else {
         launch_gather_soa<T, D>(pts, n, inrow_of_pos, soa, ld, s);
     }
@@ -479,7 +482,6 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         ba.jobs = jobs;
         ba.job_of_cell = job_of_cell;
         ba.rcenter = rcenter;
-        launch_box_reduce<T, D>(ba, nullptr, nullptr, s);
         launch_box_tight<T, D>(ba, btight, &counters[5], s);
         a.box_of_slot = box_of_slot;
         a.btight = btight;
@@ -488,6 +490,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         a.bcell = bcell;
         a.bbox = bbox;
         a.sparse_core = ctx.sparse_core;
+        a.bmincore = bmincore;
+        fill_i32(bmincore, INT_MAX, nbox, s);
         a.ditems = ditems;
         a.nditems = nditems;
         a.sitems = sitems;
@@ -504,7 +508,6 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         box_union = hc[5] == 0;  // every box of two or more rows is tight
     }
     if (box_union) {
-        launch_box_reduce<T, D>(ba, core, bmincore, s);
         // compmin holds the hook forest's roots until the clusters are known
         tm.mark(M_HOOK0);
         launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, s);

@@ -48,6 +48,11 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 
+// sorted position of a slot
+__device__ __forceinline__ int32_t slot_pos(const uint32_t* __restrict__ pos_of_slot, int32_t s) {
+    return pos_of_slot ? static_cast<int32_t>(pos_of_slot[s]) : s;
+}
+
 struct Chunk {
     int32_t u;       // occupied cell
     int32_t first;   // first row of the chunk
@@ -183,6 +188,19 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
     (void)ncore;
+    if (a.bmincore) {
+        // smallest core position per box: a box inside this chunk is stored
+        // by its first lane, one that extends past it is merged atomically
+        const int32_t b = valid ? a.box_of_slot[p] : -1 - lane_id();
+        const int32_t m = run_suffix_min<int32_t>(is_core ? slot_pos(a.pos_of_slot, p) : INT_MAX, b);
+        const bool first = run_head(b);  // every lane takes part in the shuffles
+        if (valid && first && m != INT_MAX) {
+            if (a.bstart[b] >= c.first && a.bstart[b + 1] <= c.first + kWave)
+                a.bmincore[b] = m;
+            else
+                atomicMin(&a.bmincore[b], m);
+        }
+    }
 }
 
 // Thread-per-row core counts for the rows of sparse cells (a.sitems): the
@@ -231,6 +249,7 @@ __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
             }
             is_core = cnt >= min_pts;
             core[p] = is_core ? 1 : 0;
+            if (is_core && a.bmincore) atomicMin(&a.bmincore[b], slot_pos(a.pos_of_slot, p));
         }
     }
     (void)ncore;
@@ -609,9 +628,6 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
 // the cores of the clusters they touch, by in-place relaxation to the fixed
 // point.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int32_t slot_pos(const uint32_t* __restrict__ pos_of_slot, int32_t s) {
-    return pos_of_slot ? static_cast<int32_t>(pos_of_slot[s]) : s;
-}
 
 // clusters touched by the ambiguous border rows -> involved[cluster] = 1
 template <typename T, int D>

@@ -210,4 +210,34 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
     }
 }
 
+// Runs of equal `key` over consecutive lanes (boxes are contiguous slot
+// ranges): afterwards every lane holds the minimum (maximum) of v over the
+// lanes from itself to the end of its run.
+template <typename U>
+__device__ __forceinline__ U run_suffix_min(U v, int32_t key) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const U w = __shfl_down(v, o, 64);
+        const int32_t k = __shfl_down(key, o, 64);
+        if (__lane_id() + o < 64 && k == key) v = w < v ? w : v;
+    }
+    return v;
+}
+template <typename U>
+__device__ __forceinline__ U run_suffix_max(U v, int32_t key) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const U w = __shfl_down(v, o, 64);
+        const int32_t k = __shfl_down(key, o, 64);
+        if (__lane_id() + o < 64 && k == key) v = w > v ? w : v;
+    }
+    return v;
+}
+
+// True on the lane holding the first slot of its run of equal `key`.
+__device__ __forceinline__ bool run_head(int32_t key) {
+    const int32_t prev = __shfl_up(key, 1, 64);
+    return __lane_id() == 0 || prev != key;
+}
+
 }  // namespace hpd
