@@ -240,8 +240,9 @@ __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restr
 }
 
 // Box-level hook: every box with cores points at some box of a smaller
-// index holding a core within eps of one of its cores (found by the box's
-// chunks, each stopping at its first hit; any such box will do).  parent
+// index holding a core within eps of one of its cores (searched from the
+// box's first 64-row chunk, stopping at the first hit; any such box will
+// do, and a box without one stays a root).  parent
 // stays <= the box index, so the hooks form a forest whose trees are
 // connected sets of boxes.
 template <typename T, int D>
@@ -259,6 +260,9 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     const int2 it = bitems[wid];
     const int32_t b = uniform(it.x);
     if (bmincore[b] == INT_MAX) return;
+    // one hook per box is enough: its first chunk looks for it (a missed
+    // link is found by the exact union later)
+    if (uniform(it.y) != uniform(bstart[b])) return;
     const int32_t end = uniform(bstart[b + 1]);
     const int32_t p = it.y + lane_id();
     const bool pcore = p < end && core[p];
@@ -314,6 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_box_root_join(
     const int2 it = bitems[wid];
     const int32_t b = uniform(it.x);
     if (rsnap[b] != b) return;  // not a root (or no cores)
+    if (uniform(it.y) != uniform(bstart[b])) return;  // the box's first chunk only
     const int32_t end = uniform(bstart[b + 1]);
     const int32_t p = it.y + lane_id();
     const bool pcore = p < end && core[p];

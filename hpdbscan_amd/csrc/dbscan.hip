@@ -364,6 +364,24 @@ __global__ __launch_bounds__(kThreads) void k_label_cores(
     }
 }
 
+// Box mode: core labels one thread per slot (the cluster of a core is its
+// box's); cmin / cmax are not needed there.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
+    int64_t n, const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ job_of_cell,
+    const uint8_t* __restrict__ core, const int32_t* __restrict__ box_of_slot,
+    const int32_t* __restrict__ bcell, const int32_t* __restrict__ bcluster,
+    const uint32_t* __restrict__ inrow, int64_t* __restrict__ labels,
+    int32_t* __restrict__ rootslot) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (p >= n || !core[p]) return;
+    const int32_t b = box_of_slot[p];
+    const int32_t r = bcluster[b];
+    rootslot[p] = r;
+    const int64_t first_pos = jobs[job_of_cell[bcell[b]]].row_begin;
+    labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
+}
+
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_union(const DbscanArgs<T> a,
                                                     const uint8_t* __restrict__ core,
@@ -860,6 +878,13 @@ void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
                         int32_t* cmax, int64_t* labels, hipStream_t s) {
     if (a.n <= 0) return;
+    if (bcluster && a.bcell) {
+        hipLaunchKernelGGL(k_label_cores_boxes<T>, dim3(blocks_for(a.n, kThreads)), dim3(kThreads),
+                           0, s, a.n, a.jobs, a.job_of_cell, core, a.box_of_slot, a.bcell, bcluster,
+                           a.inrow, labels, rootslot);
+        HPD_HIP(hipGetLastError());
+        return;
+    }
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
                        cmax, -1, static_cast<int64_t>(a.m));

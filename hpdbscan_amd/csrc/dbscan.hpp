@@ -66,7 +66,8 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 // Core labels: core slots get -(cluster + 1), where cluster comes from
 // bcluster (box mode, bcluster != null) or compmin[find(parent, slot)];
 // rootslot[slot] = cluster, cmin / cmax = smallest / largest cluster per
-// occupied cell.  Labels go to input order through a.inrow.
+// occupied cell (not in box mode, where the border pass reads bcluster).
+// Labels go to input order through a.inrow.
 // rootslot: ld entries.
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
