@@ -30,6 +30,7 @@
 #include "boxes.hpp"
 #include "common.hpp"
 #include "grid.hpp"
+#include "index.hpp"
 #include "internal.hpp"
 #include "visit.hpp"
 
@@ -599,15 +600,6 @@ __global__ __launch_bounds__(kThreads) void k_box_items_count(const int32_t* __r
     cnt[b] = static_cast<uint32_t>((bstart[b + 1] - bstart[b] + kWave - 1) / kWave);
 }
 
-__global__ __launch_bounds__(kThreads) void k_box_items_fill(const int32_t* __restrict__ bstart,
-                                                             int32_t nbox,
-                                                             const uint32_t* __restrict__ off,
-                                                             int2* __restrict__ items) {
-    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (b >= nbox) return;
-    uint32_t w = off[b];
-    for (int32_t s = bstart[b]; s < bstart[b + 1]; s += kWave) items[w++] = make_int2(b, s);
-}
 
 
 __global__ __launch_bounds__(kThreads) void k_fill_i32(int32_t* __restrict__ a, int32_t v,
@@ -684,10 +676,9 @@ void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32
     HPD_HIP(hipGetLastError());
 }
 
-void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off, int2* items,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_box_items_fill, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       bstart, nbox, off, items);
+void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off,
+                           int32_t nitems, int2* items, hipStream_t s) {
+    launch_expand_chunks(off, nbox, bstart, nitems, items, s);
     HPD_HIP(hipGetLastError());
 }
 

@@ -56,8 +56,8 @@ void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* c
 // cnt[b] = 64-slot chunks of box b, scanned in place; *total_dev = chunks
 void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32_t* scan_tmp,
                       uint32_t* total_dev, hipStream_t s);
-void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off, int2* items,
-                           hipStream_t s);
+void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off,
+                           int32_t nitems, int2* items, hipStream_t s);
 
 // btight[b]: bit 0 tight, bit 1 tight with >= minPts rows; *nloose counts
 // boxes of two or more rows that are not tight

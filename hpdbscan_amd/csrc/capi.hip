@@ -444,8 +444,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
     launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
                                rcenter, s);
-    launch_work_items(ustart, m, nchunks, items, s);
-    if (boxes) launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, sitems, s);
+    launch_work_items(ustart, m, nchunks, nitems, items, s);
+    if (boxes)
+        launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, nditems, sitems, s);
 
     DbscanArgs<T> a;
     a.soa = soa;
@@ -468,7 +469,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     bool box_union = false;
     if constexpr (kFloat) {
         launch_cell_boxes(bcell, nbox, m, cbox, s);
-        launch_box_items_fill(bstart, nbox, bcnt, bitems, s);
+        launch_box_items_fill(bstart, nbox, bcnt, nbitems, bitems, s);
         ba.soa = soa;
         ba.ld = ld;
         ba.nbox = nbox;

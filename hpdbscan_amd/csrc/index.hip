@@ -197,14 +197,22 @@ __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict_
     nchunks[u] = static_cast<uint32_t>((c + kWave - 1) / kWave);
 }
 
-__global__ __launch_bounds__(256) void k_work_items(const int32_t* __restrict__ ustart, int32_t m,
-                                                    const uint32_t* __restrict__ coff,
-                                                    int2* __restrict__ items) {
-    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (u >= m) return;
-    const int32_t s = ustart[u], e = ustart[u + 1];
-    uint32_t w = coff[u];
-    for (int32_t p = s; p < e; p += kWave) items[w++] = make_int2(u, p);
+// items[w] = (group, first row) of the w-th 64-row chunk, one thread per
+// chunk: the group is the last one whose chunk offset is <= w (binary search
+// over the exclusive scan off[0 .. ngroups]; empty groups are skipped)
+__global__ __launch_bounds__(256) void k_expand_chunks(const uint32_t* __restrict__ off,
+                                                       int32_t ngroups,
+                                                       const int32_t* __restrict__ start,
+                                                       int32_t total, int2* __restrict__ items) {
+    const int32_t w = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (w >= total) return;
+    int32_t lo = 0, hi = ngroups;  // off[lo] <= w < off[hi]
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= static_cast<uint32_t>(w)) lo = mid;
+        else hi = mid;
+    }
+    items[w] = make_int2(lo, start[lo] + kWave * (w - static_cast<int32_t>(off[lo])));
 }
 
 // Per cell: 64-row chunks if the cell has more than `sparse_max` rows
@@ -233,10 +241,9 @@ __global__ __launch_bounds__(256) void k_split_items(const int32_t* __restrict__
     if (e - s <= sparse_max) {
         uint32_t w = soff[u];
         for (int32_t p = s; p < e; ++p) sitems[w++] = make_int2(p, u);
-    } else {
-        uint32_t w = doff[u];
-        for (int32_t p = s; p < e; p += kWave) ditems[w++] = make_int2(u, p);
     }
+    (void)doff;
+    (void)ditems;
 }
 
 }  // namespace
@@ -249,9 +256,19 @@ void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, u
 }
 
 void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
-                        const uint32_t* soff, int2* ditems, int2* sitems, hipStream_t s) {
+                        const uint32_t* soff, int2* ditems, int32_t nditems, int2* sitems,
+                        hipStream_t s) {
     hipLaunchKernelGGL(k_split_items, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
                        sparse_max, doff, soff, ditems, sitems);
+    launch_expand_chunks(doff, m, ustart, nditems, ditems, s);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_expand_chunks(const uint32_t* off, int32_t ngroups, const int32_t* start,
+                          int32_t total, int2* items, hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_expand_chunks, dim3(blocks_for(total, 256)), dim3(256), 0, s, off, ngroups,
+                       start, total, items);
     HPD_HIP(hipGetLastError());
 }
 
@@ -291,11 +308,9 @@ void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hi
     HPD_HIP(hipGetLastError());
 }
 
-void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_work_items, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m, coff,
-                       items);
-    HPD_HIP(hipGetLastError());
+void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int32_t nitems,
+                       int2* items, hipStream_t s) {
+    launch_expand_chunks(coff, m, ustart, nitems, items, s);
 }
 
 #define HPD_INST_INDEX(K)                                                                     \

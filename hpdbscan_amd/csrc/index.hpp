@@ -40,8 +40,14 @@ void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hi
 void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, uint32_t* dcnt,
                          uint32_t* scnt, hipStream_t s);
 void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
-                        const uint32_t* soff, int2* ditems, int2* sitems, hipStream_t s);
-void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int2* items,
-                       hipStream_t s);
+                        const uint32_t* soff, int2* ditems, int32_t nditems, int2* sitems,
+                        hipStream_t s);
+
+// items[w] = (g, start[g] + 64 * (w - off[g])) for the `total` 64-row chunks
+// of ngroups groups (off: exclusive scan of chunk counts, ngroups + 1 entries)
+void launch_expand_chunks(const uint32_t* off, int32_t ngroups, const int32_t* start,
+                          int32_t total, int2* items, hipStream_t s);
+void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int32_t nitems,
+                       int2* items, hipStream_t s);
 
 }  // namespace hpd
