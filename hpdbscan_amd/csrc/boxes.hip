@@ -47,62 +47,14 @@ constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
-template <typename T>
-__device__ __forceinline__ void scaled(T x, T mn, float rcp_f, double rcp_d, double* out_v);
-
-template <>
-__device__ __forceinline__ void scaled<float>(float x, float mn, float rcp_f, double, double* v) {
-    *v = static_cast<double>((x - mn) * rcp_f);
-}
-template <>
-__device__ __forceinline__ void scaled<double>(double x, double mn, float, double rcp_d, double* v) {
-    *v = (x - mn) * rcp_d;
-}
-
-// ---------------------------------------------------------------------------
-// box keys: (cell index << sub_bits) | sub-box digit
-// ---------------------------------------------------------------------------
-template <typename T, int D, typename K>
-__global__ __launch_bounds__(kThreads) void k_box_keys(const T* __restrict__ pts, int64_t n,
-                                                       const uint32_t* __restrict__ inrow_of_pos,
-                                                       const uint32_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ scan,
-                                                       const JobParams<T>* __restrict__ jobs,
-                                                       int32_t njobs, int k, int sub_bits,
-                                                       K* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals) {
-    const int64_t pos = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (pos >= n) return;
-    // sorted positions of a job are its own row range, so the job follows
-    // from the position
-    const JobParams<T>& jp = jobs[job_of_row(jobs, njobs, pos)];
-    const int64_t row = inrow_of_pos[pos];
-    uint32_t sub = 0, mul = 1;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        double v;
-        scaled<T>(pts[row * D + j], jp.minv[j], jp.rcp_f, jp.rcp_d, &v);
-        // v is the value the cell index floors (held exactly in double)
-        const double c = floor(v);
-        long long sd = static_cast<long long>(floor(v * k)) - static_cast<long long>(c) * k;
-        sd = sd < 0 ? 0 : (sd >= k ? k - 1 : sd);
-        sub += static_cast<uint32_t>(sd) * mul;
-        mul *= static_cast<uint32_t>(k);
-    }
-    const uint64_t u = static_cast<uint64_t>(scan[pos] + flags[pos] - 1u);  // occupied-cell index
-    keys[pos] = static_cast<K>((u << sub_bits) | sub);
-    vals[pos] = static_cast<uint32_t>(pos);
-}
-
-// rows in box order: coordinates (SoA) and input row; with it the bounding
-// box of every box (ordered keys, pre-filled with +inf / -inf keys): a box
+// rows in slot order: coordinates (SoA); with it the bounding box of every
+// box (ordered keys, pre-filled with +inf / -inf keys): a box
 // inside the wave's 64 slots is stored by its first lane, one that crosses
 // the wave's edge is merged with atomics
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_gather_boxes(
-    const T* __restrict__ pts, int64_t n, const uint32_t* __restrict__ pos_of_slot,
-    const uint32_t* __restrict__ inrow_of_pos, T* __restrict__ soa, int64_t ld,
-    uint32_t* __restrict__ inrow_of_slot, const int32_t* __restrict__ box_of_slot,
+    const T* __restrict__ pts, int64_t n, const uint32_t* __restrict__ inrow_of_slot,
+    T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ box_of_slot,
     const int32_t* __restrict__ bstart, OrdKey<T>* __restrict__ bbox) {
     using U = OrdKey<T>;
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -110,8 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_boxes(
     int32_t b = -1 - lane_id();  // distinct keys for lanes past the end
     T x[D];
     if (valid) {
-        const uint32_t row = inrow_of_pos[pos_of_slot[s]];
-        inrow_of_slot[s] = row;
+        const uint32_t row = inrow_of_slot[s];
         b = box_of_slot[s];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -157,7 +108,9 @@ template <typename K>
 __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ keys, int64_t n,
                                                         const uint32_t* __restrict__ flags,
                                                         const uint32_t* __restrict__ scan,
-                                                        int sub_bits, int32_t* __restrict__ bstart,
+                                                        const uint32_t* __restrict__ cflags,
+                                                        const uint32_t* __restrict__ cscan,
+                                                        int32_t* __restrict__ bstart,
                                                         int32_t* __restrict__ bcell,
                                                         int32_t* __restrict__ box_of_slot,
                                                         const uint32_t* __restrict__ nbox_dev,
@@ -169,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
         box_of_slot[i] = static_cast<int32_t>(b);
         if (flags[i]) {
             bstart[b] = static_cast<int32_t>(i);
-            bcell[b] = static_cast<int32_t>(static_cast<uint64_t>(keys[i]) >> sub_bits);
+            bcell[b] = static_cast<int32_t>(cscan[i] + cflags[i] - 1u);  // occupied-cell index
             // empty bounding box: min keys all ones, max keys zero
             if (bbox) {
                 const int64_t k0 = static_cast<int64_t>(b) * bbox_keys;
@@ -571,14 +524,57 @@ __global__ __launch_bounds__(kThreads) void k_box_union(
     }
 }
 
+// Cluster ids.  A sorted position orders rows by (occupied cell, input
+// row), so the smallest position of a component's cores is the smallest
+// key (cell << 32 | row) of its boxes' smallest core rows (bmincore holds
+// the input row); the position itself is the cell's first slot plus the
+// number of the cell's rows with a smaller input row (every box of the
+// cell holds its rows in input order, so a binary search per box counts).
 __global__ __launch_bounds__(kThreads) void k_box_compmin(int32_t nbox,
                                                           const int32_t* __restrict__ bmincore,
+                                                          const int32_t* __restrict__ bcell,
                                                           int32_t* bparent,
-                                                          int32_t* __restrict__ compmin) {
+                                                          unsigned long long* __restrict__ cmin) {
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox || bmincore[b] == INT_MAX) return;
     const int32_t r = uf_find(bparent, b);
-    atomicMin(&compmin[r], bmincore[b]);
+    const unsigned long long key = (static_cast<unsigned long long>(bcell[b]) << 32) |
+                                   static_cast<uint32_t>(bmincore[b]);
+    atomicMin(&cmin[r], key);
+}
+
+// number of entries < row in the ascending v[lo, hi)
+__device__ __forceinline__ int32_t count_below(const uint32_t* __restrict__ v, int32_t lo,
+                                               int32_t hi, uint32_t row) {
+    const int32_t first = lo;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (v[mid] < row) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - first;
+}
+
+// position of every component root's smallest core (one wave per box;
+// non-roots leave at once)
+__global__ __launch_bounds__(kThreads) void k_box_resolve(
+    int32_t nbox, const int32_t* __restrict__ bparent, const unsigned long long* __restrict__ cmin,
+    const int32_t* __restrict__ cbox, const int32_t* __restrict__ bstart,
+    const uint32_t* __restrict__ inrow, const int32_t* __restrict__ ustart,
+    int32_t* __restrict__ pos) {
+    const int32_t r = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                              static_cast<int32_t>(threadIdx.x >> 6));
+    if (r >= nbox || bparent[r] != r) return;
+    const unsigned long long key = cmin[r];
+    if (key == ~0ull) return;  // no cores
+    const int32_t u = static_cast<int32_t>(key >> 32);
+    const uint32_t row = static_cast<uint32_t>(key & 0xffffffffu);
+    int32_t below = 0;
+    for (int32_t b2 = uniform(cbox[u]) + lane_id(); b2 < cbox[u + 1]; b2 += kWave)
+        below += count_below(inrow, bstart[b2], bstart[b2 + 1], row);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);
+    if (lane_id() == 0) pos[r] = ustart[u] + below;
 }
 
 // cluster id (smallest core position of the component) per box with cores
@@ -601,6 +597,12 @@ __global__ __launch_bounds__(kThreads) void k_box_items_count(const int32_t* __r
 }
 
 
+
+__global__ __launch_bounds__(kThreads) void k_fill_u64(unsigned long long* __restrict__ a,
+                                                       unsigned long long v, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) a[i] = v;
+}
 
 __global__ __launch_bounds__(kThreads) void k_fill_i32(int32_t* __restrict__ a, int32_t v,
                                                        int64_t n) {
@@ -625,39 +627,30 @@ int box_sub_bits(int d) {
     return mx == 0 ? 0 : 64 - __builtin_clzll(mx);
 }
 
-template <typename T, int D, typename K>
-void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
-                     const uint32_t* scan, const JobParams<T>* jobs, int32_t njobs, K* keys,
-                     uint32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL((k_box_keys<T, D, K>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                       pts, n, inrow_of_pos, flags, scan, jobs, njobs, box_split(D),
-                       box_sub_bits(D), keys, vals);
-    HPD_HIP(hipGetLastError());
-}
 
 template <typename T, int D>
-void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
-                         const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
-                         const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
+void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot, T* soa,
+                         int64_t ld, const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
                          hipStream_t s) {
     using U = OrdKey<T>;
+    if (n <= 0) return;
     hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                       pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot, box_of_slot,
-                       bstart, static_cast<U*>(bbox));
+                       pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, static_cast<U*>(bbox));
     HPD_HIP(hipGetLastError());
 }
 
 template <typename K>
-void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
-                      int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
-                      void* bbox, int bbox_keys, int key_bytes, hipStream_t s) {
+void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const uint32_t* cscan,
+                      uint32_t* flags, uint32_t* scan, uint32_t* scan_tmp, int32_t* bstart,
+                      int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev, void* bbox,
+                      int bbox_keys, int key_bytes, hipStream_t s) {
     hipLaunchKernelGGL(k_box_flags<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
                        flags);
-    exclusive_scan_u32(flags, flags + n, n, scan_tmp, nbox_dev, s);
+    exclusive_scan_u32(flags, scan, n, scan_tmp, nbox_dev, s);
     hipLaunchKernelGGL(k_box_table<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
-                       static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(flags + n),
-                       sub_bits, bstart, bcell, box_of_slot, static_cast<const uint32_t*>(nbox_dev),
-                       bbox, bbox_keys, key_bytes);
+                       static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(scan),
+                       cflags, cscan, bstart, bcell, box_of_slot,
+                       static_cast<const uint32_t*>(nbox_dev), bbox, bbox_keys, key_bytes);
     HPD_HIP(hipGetLastError());
 }
 
@@ -757,29 +750,63 @@ void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     HPD_HIP(hipGetLastError());
 }
 
-void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
+void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* bcell,
+                         const int32_t* cbox, const int32_t* bstart, const uint32_t* inrow,
+                         const int32_t* ustart, int32_t* bparent, unsigned long long* cmin,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       compmin, INT_MAX, static_cast<int64_t>(nbox));
+    if (nbox <= 0) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, cmin,
+                       ~0ull, static_cast<int64_t>(nbox));
     hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       nbox, bmincore, bparent, compmin);
+                       nbox, bmincore, bcell, bparent, cmin);
+    hipLaunchKernelGGL(k_box_resolve, dim3(blocks_for(nbox, kWavesPerBlock)), dim3(kThreads), 0, s,
+                       nbox, static_cast<const int32_t*>(bparent),
+                       static_cast<const unsigned long long*>(cmin), cbox, bstart, inrow, ustart,
+                       compmin);
     hipLaunchKernelGGL(k_box_cluster, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
                        nbox, bmincore, bparent, static_cast<const int32_t*>(compmin), bcluster);
     HPD_HIP(hipGetLastError());
 }
 
+// Sorted positions of all slots (for the slot-level union and the
+// one-thread border rule): a stable sort of the slots by occupied cell in
+// input order gives the (cell, input row) order.
+__global__ __launch_bounds__(kThreads) void k_pos_keys(int64_t n, const int32_t* __restrict__ box_of_slot,
+                                                       const int32_t* __restrict__ bcell,
+                                                       const uint32_t* __restrict__ inrow,
+                                                       uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t r = inrow[s];
+    keys[r] = static_cast<uint32_t>(bcell[box_of_slot[s]]);
+    vals[r] = static_cast<uint32_t>(s);
+}
+
+__global__ __launch_bounds__(kThreads) void k_pos_scatter(int64_t n,
+                                                          const uint32_t* __restrict__ slot_at,
+                                                          uint32_t* __restrict__ pos_of_slot) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) pos_of_slot[slot_at[i]] = static_cast<uint32_t>(i);
+}
+
+void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const int32_t* bcell,
+                      const uint32_t* inrow, uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b,
+                      uint32_t* vals_b, uint32_t* hist, uint32_t* tmp, uint32_t* pos_of_slot,
+                      hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pos_keys, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n,
+                       box_of_slot, bcell, inrow, keys_a, vals_a);
+    const int bits = m > 1 ? 64 - __builtin_clzll(static_cast<unsigned long long>(m - 1)) : 0;
+    const bool in_b = radix_sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
+    hipLaunchKernelGGL(k_pos_scatter, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n,
+                       static_cast<const uint32_t*>(in_b ? vals_b : vals_a), pos_of_slot);
+    HPD_HIP(hipGetLastError());
+}
+
 #define HPD_INST_BOX_KEYS(T, D)                                                                   \
-    template void launch_box_keys<T, D, uint32_t>(const T*, int64_t, const uint32_t*,             \
-                                                  const uint32_t*, const uint32_t*,               \
-                                                  const JobParams<T>*, int32_t, uint32_t*,        \
-                                                  uint32_t*, hipStream_t);                        \
-    template void launch_box_keys<T, D, uint64_t>(const T*, int64_t, const uint32_t*,             \
-                                                  const uint32_t*, const uint32_t*,               \
-                                                  const JobParams<T>*, int32_t, uint64_t*,        \
-                                                  uint32_t*, hipStream_t);                        \
-    template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, const uint32_t*,  \
-                                            T*, int64_t, uint32_t*, const int32_t*,               \
-                                            const int32_t*, void*, hipStream_t);                  \
+    template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, T*, int64_t,      \
+                                            const int32_t*, const int32_t*, void*, hipStream_t);  \
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
@@ -805,10 +832,12 @@ void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent
 HPD_INST_BOX_ALL_D(float)
 HPD_INST_BOX_ALL_D(double)
 
-template void launch_box_table<uint32_t>(const uint32_t*, int64_t, int, uint32_t*, uint32_t*,
+template void launch_box_table<uint32_t>(const uint32_t*, int64_t, const uint32_t*,
+                                         const uint32_t*, uint32_t*, uint32_t*, uint32_t*,
                                          int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
                                          hipStream_t);
-template void launch_box_table<uint64_t>(const uint64_t*, int64_t, int, uint32_t*, uint32_t*,
+template void launch_box_table<uint64_t>(const uint64_t*, int64_t, const uint32_t*,
+                                         const uint32_t*, uint32_t*, uint32_t*, uint32_t*,
                                          int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
                                          hipStream_t);
 

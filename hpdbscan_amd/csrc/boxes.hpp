@@ -23,33 +23,30 @@ struct BoxArgs {
     const int32_t* cbox = nullptr;     // first box of every cell, [m + 1]
     const int2* bitems = nullptr;      // (box, first slot) per 64-slot chunk
     int32_t nbitems = 0;
-    const uint32_t* pos_of_slot = nullptr;
     void* bbox = nullptr;              // ordered-key min/max per box and dim
     const JobParams<T>* jobs = nullptr;
     const int32_t* job_of_cell = nullptr;
     const int32_t* rcenter = nullptr;  // index of every cell's own neighbour row
 };
 
-// key = (occupied-cell index << sub_bits) | sub-box digit, val = position
-template <typename T, int D, typename K>
-void launch_box_keys(const T* pts, int64_t n, const uint32_t* inrow_of_pos, const uint32_t* flags,
-                     const uint32_t* scan, const JobParams<T>* jobs, int32_t njobs, K* keys,
-                     uint32_t* vals, hipStream_t s);
-
+// coordinates in slot order (soa[j * ld + s] = pts[inrow_of_slot[s] * D + j])
+// and the bounding boxes (bbox, initialised by launch_box_table)
 template <typename T, int D>
-// also the bounding boxes (bbox, initialised by launch_box_table)
-void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* pos_of_slot,
-                         const uint32_t* inrow_of_pos, T* soa, int64_t ld, uint32_t* inrow_of_slot,
-                         const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
+void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot, T* soa,
+                         int64_t ld, const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
                          hipStream_t s);
 
-// flags: 2n + 2 words; bstart: nbox + 1; writes *nbox_dev; bbox (when not
-// null, bbox_keys keys of key_bytes each per box) is set to empty boxes
+// Boxes of keys sorted by (cell, sub-box digit): runs of equal keys.
+// cflags / cscan are the cell-run flags and their exclusive scan (they give
+// each box's occupied-cell index); flags and scan (n words each) and
+// scan_tmp are scratch; bstart: nbox + 1 entries; writes *nbox_dev; bbox
+// (when not null, bbox_keys keys of key_bytes each per box) is set to
+// empty boxes.
 template <typename K>
-void launch_box_table(const K* keys, int64_t n, int sub_bits, uint32_t* flags, uint32_t* scan_tmp,
-                      int32_t* bstart, int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev,
-                      void* bbox, int bbox_keys, int key_bytes, hipStream_t s);
-
+void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const uint32_t* cscan,
+                      uint32_t* flags, uint32_t* scan, uint32_t* scan_tmp, int32_t* bstart,
+                      int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev, void* bbox,
+                      int bbox_keys, int key_bytes, hipStream_t s);
 void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
                        hipStream_t s);
 
@@ -70,7 +67,9 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
 //                        -1 for boxes without cores)
 //   launch_box_join      union of the box pairs the snapshot does not join
 //   launch_box_clusters  bcluster[b] = smallest core position of b's
-//                        component (-1 without cores); uses compmin (nbox)
+//                        component (-1 without cores); bmincore holds the
+//                        smallest core input row per box; uses cmin and
+//                        compmin (nbox each) as scratch
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
@@ -96,7 +95,17 @@ void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint3
 // p[0, n) = v
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 
-void launch_box_clusters(int32_t nbox, const int32_t* bmincore, int32_t* bparent,
+void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* bcell,
+                         const int32_t* cbox, const int32_t* bstart, const uint32_t* inrow,
+                         const int32_t* ustart, int32_t* bparent, unsigned long long* cmin,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s);
 
+// pos_of_slot[s] = sorted position of slot s (the reference's order: cell,
+// then input row), by a stable radix sort of the slots by occupied cell in
+// input order; keys_a / vals_a / keys_b / vals_b (n words each), hist and
+// tmp are scratch sized as for radix_sort_pairs<uint32_t>
+void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const int32_t* bcell,
+                      const uint32_t* inrow, uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b,
+                      uint32_t* vals_b, uint32_t* hist, uint32_t* tmp, uint32_t* pos_of_slot,
+                      hipStream_t s);
 }  // namespace hpd

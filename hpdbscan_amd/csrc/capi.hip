@@ -269,13 +269,13 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     const size_t hist_words = radix_hist_words(n);
     const size_t tmp_words = scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2)));
     size_t b = 0;
-    b += 4 * 8 * nn + 4 * 4 * nn;              // keys / values of both sorts
+    b += 2 * 8 * nn + 2 * 4 * nn;              // keys / values of the sort
     b += 4 * hist_words + 4 * tmp_words;       // radix histogram, scan scratch
     b += sizeof(T) * L * D;                    // coordinates in slot order
     b += 4 * (2 * nn + 2);                     // run flags + scan
     b += 8 * nn + 3 * 4 * (nn + 1);            // ucell, ustart, nranges, nchunks
     b += L + 3 * 4 * L + 3 * 4 * nn;           // core, parent, rsnap, rootslot, cmin/cmax/rcenter
-    b += 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // box tables
+    b += 8 * nn + 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // scratch64, positions, box tables
     b += 2 * D * 8 * nn;                       // box bounding boxes
     const size_t nj = static_cast<size_t>(njobs);
     const size_t mm_blocks = nn / static_cast<size_t>(minmax_rows_per_block()) + nj + 1;
@@ -288,9 +288,8 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
 
 template <typename T, int D, typename K>
 static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>* jobs,
-                      int32_t njobs, int key_bits, int64_t* labels, hipStream_t s,
+                      int32_t njobs, int key_bits, bool boxes, int64_t* labels, hipStream_t s,
                       PhaseTimer& tm) {
-    constexpr bool kFloat = std::is_floating_point<T>::value;
     Arena& A = ctx.main;
     const size_t nn = static_cast<size_t>(n);
     const int64_t ld = (n + 15) & ~int64_t(15);  // rows are read in aligned batches of <= 16
@@ -321,41 +320,41 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     HPD_HIP(hipMemsetAsync(dstats, 0, 16 * sizeof(unsigned long long), s));
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.host_scratch);
 
-    // cell ids (spatial_index.h:144-165), offset by the job's key base
-    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, keys_a, vals_a, s);
+    // cell ids (spatial_index.h:144-165), offset by the job's key base; in
+    // box mode followed by the row's sub-box digit, so that one stable sort
+    // orders the rows by cell, then sub-box, then input row
+    const int sub_bits = boxes ? box_sub_bits(D) : 0;
+    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, box_split(D), sub_bits, keys_a, vals_a, s);
     tm.mark(M_GRID);
 
-    // stable sort by cell (spatial_index.h:167-216): position -> input row;
-    // jobs stay contiguous and in order because their key ranges are
-    const bool in_b = radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, key_bits, hist, tmp, s);
+    // stable sort (spatial_index.h:167-216): slot -> input row; jobs stay
+    // contiguous and in order because their key ranges are
+    const bool in_b =
+        radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, key_bits + sub_bits, hist, tmp, s);
     K* keys = in_b ? keys_b : keys_a;
-    uint32_t* inrow_of_pos = in_b ? vals_b : vals_a;
+    uint32_t* inrow_of_slot = in_b ? vals_b : vals_a;
+    K* keys_free = in_b ? keys_a : keys_b;
+    uint32_t* vals_free = in_b ? vals_a : vals_b;
 
-    // occupied cells (spatial_index.h:167-182)
-    launch_cells<K>(keys, n, flags, tmp, ucell, ustart, &counters[0], s);
+    // occupied cells (spatial_index.h:167-182); their order and the order of
+    // input rows inside them are the reference's sorted order
+    launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &counters[0], s);
     HPD_HIP(hipMemcpyAsync(hc, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
     const int32_t m = static_cast<int32_t>(hc[0]);
 
-    // slot order: in box mode, rows regrouped by sub-box inside each cell
-    const bool boxes = kFloat;
-    const uint32_t* pos_of_slot = nullptr;
-    uint32_t* inrow_of_slot = inrow_of_pos;
+    // box mode: slots are rows sorted by cell, sub-box, input row; the
+    // reference's sorted positions are only materialised where needed
+    uint32_t* pos_of_slot = nullptr;
     int32_t *box_of_slot = nullptr, *bstart = nullptr, *bcell = nullptr, *cbox = nullptr;
     uint32_t* bcnt = nullptr;
     int32_t *bmincore = nullptr, *bparent = nullptr, *compmin = nullptr, *bcluster = nullptr;
     uint8_t* btight = nullptr;
     void* bbox = nullptr;
-    uint64_t* box_sort_keys = nullptr;  // n words, unused after the box sort
-    if constexpr (kFloat) {
-        const int sub_bits = box_sub_bits(D);
-        const int bits2 = bits_for(static_cast<uint64_t>(m - 1)) + sub_bits;
-        uint64_t* k2a = A.take_n<uint64_t>(nn);
-        box_sort_keys = k2a;
-        uint64_t* k2b = A.take_n<uint64_t>(nn);
-        uint32_t* v2a = A.take_n<uint32_t>(nn);
-        uint32_t* v2b = A.take_n<uint32_t>(nn);
-        inrow_of_slot = A.take_n<uint32_t>(nn);
+    uint64_t* scratch64 = nullptr;  // n words: open-pair lists, cluster keys, position sort
+    if (boxes) {
+        scratch64 = A.take_n<uint64_t>(nn);
+        pos_of_slot = A.take_n<uint32_t>(nn);
         box_of_slot = A.take_n<int32_t>(nn);
         bstart = A.take_n<int32_t>(nn + 1);
         bcell = A.take_n<int32_t>(nn);
@@ -367,34 +366,13 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         bcluster = A.take_n<int32_t>(nn);
         btight = A.take_n<uint8_t>(nn);
         bbox = A.take(2 * D * sizeof(T) * nn);
-        const uint32_t* slot_pos;
-        const void* sorted2;
-        if (bits2 <= 32) {
-            uint32_t* ka = reinterpret_cast<uint32_t*>(k2a);
-            uint32_t* kb = reinterpret_cast<uint32_t*>(k2b);
-            launch_box_keys<T, D, uint32_t>(pts, n, inrow_of_pos, flags, flags + n, jobs, njobs,
-                                            ka, v2a, s);
-            const bool ib = radix_sort_pairs<uint32_t>(ka, v2a, kb, v2b, n, bits2, hist, tmp, s);
-            slot_pos = ib ? v2b : v2a;
-            sorted2 = ib ? static_cast<const void*>(kb) : static_cast<const void*>(ka);
-            launch_box_table<uint32_t>(static_cast<const uint32_t*>(sorted2), n, sub_bits, flags, tmp,
-                                       bstart, bcell, box_of_slot, &counters[3], bbox, 2 * D,
-                                       static_cast<int>(sizeof(T)), s);
-        } else {
-            launch_box_keys<T, D, uint64_t>(pts, n, inrow_of_pos, flags, flags + n, jobs, njobs,
-                                            k2a, v2a, s);
-            const bool ib = radix_sort_pairs<uint64_t>(k2a, v2a, k2b, v2b, n, bits2, hist, tmp, s);
-            slot_pos = ib ? v2b : v2a;
-            sorted2 = ib ? static_cast<const void*>(k2b) : static_cast<const void*>(k2a);
-            launch_box_table<uint64_t>(static_cast<const uint64_t*>(sorted2), n, sub_bits, flags, tmp,
-                                       bstart, bcell, box_of_slot, &counters[3], bbox, 2 * D,
-                                       static_cast<int>(sizeof(T)), s);
-        }
-        pos_of_slot = slot_pos;
-        launch_gather_boxes<T, D>(pts, n, pos_of_slot, inrow_of_pos, soa, ld, inrow_of_slot,
-                                  box_of_slot, bstart, bbox, s);
+        // box flags and their scan in the sort's free buffers
+        launch_box_table<K>(keys, n, flags, flags + n, vals_free,
+                            reinterpret_cast<uint32_t*>(keys_free), tmp, bstart, bcell, box_of_slot,
+                            &counters[3], bbox, 2 * D, static_cast<int>(sizeof(T)), s);
+        launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, s);
     } else {
-        launch_gather_soa<T, D>(pts, n, inrow_of_pos, soa, ld, s);
+        launch_gather_soa<T, D>(pts, n, inrow_of_slot, soa, ld, s);
     }
     tm.mark(M_SORT);
 
@@ -462,12 +440,12 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     a.m = m;
     a.jobs = jobs;
     a.job_of_cell = job_of_cell;
-    a.pos_of_slot = pos_of_slot;
+    a.pos_of_slot = nullptr;  // slots are positions unless boxes reorder them
     a.inrow = inrow_of_slot;
 
     BoxArgs<T> ba;
     bool box_union = false;
-    if constexpr (kFloat) {
+    if (boxes) {
         launch_cell_boxes(bcell, nbox, m, cbox, s);
         launch_box_items_fill(bstart, nbox, bcnt, nbitems, bitems, s);
         ba.soa = soa;
@@ -478,7 +456,6 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         ba.cbox = cbox;
         ba.bitems = bitems;
         ba.nbitems = nbitems;
-        ba.pos_of_slot = pos_of_slot;
         ba.bbox = bbox;
         ba.jobs = jobs;
         ba.job_of_cell = job_of_cell;
@@ -503,7 +480,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     launch_core<T, D>(a, core, &dstats[1], s);
     tm.mark(M_CORE);
 
-    if constexpr (kFloat) {
+    if (boxes) {
         HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HPD_HIP(hipStreamSynchronize(s));
         box_union = hc[5] == 0;  // every box of two or more rows is tight
@@ -513,9 +490,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         tm.mark(M_HOOK0);
         launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, s);
         tm.mark(M_HOOK1);
-        // the open pairs of every box, as a list in the (now unused)
-        // box-sort keys; offsets in bcnt (free after the box items)
-        int32_t* list = reinterpret_cast<int32_t*>(box_sort_keys);
+        // the open pairs of every box, as a list in scratch64; offsets in
+        // bcnt (free after the box items)
+        int32_t* list = reinterpret_cast<int32_t*>(scratch64);
         const uint64_t cap = 2 * static_cast<uint64_t>(nn);
         launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
         unsigned long long* hs64 = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
@@ -531,8 +508,17 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
             launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
         }
         tm.mark(M_JOIN1);
-        launch_box_clusters(ba.nbox, bmincore, bparent, compmin, bcluster, s);
+        launch_box_clusters(nbox, bmincore, bcell, cbox, bstart, inrow_of_slot, ustart, bparent,
+                            reinterpret_cast<unsigned long long*>(scratch64), compmin, bcluster, s);
     } else {
+        if (boxes) {
+            // a loose box: the slot-level union needs the sorted positions
+            launch_positions(n, m, box_of_slot, bcell, inrow_of_slot,
+                             reinterpret_cast<uint32_t*>(keys), vals_free,
+                             reinterpret_cast<uint32_t*>(keys_free),
+                             reinterpret_cast<uint32_t*>(scratch64), hist, tmp, pos_of_slot, s);
+            a.pos_of_slot = pos_of_slot;
+        }
         if (!compmin) compmin = A.take_n<int32_t>(nn);
         launch_union_points<T, D>(a, core, parent, rsnap, cmin, cmax, compmin, s);
     }
@@ -552,6 +538,13 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         // the reference's choice at one thread for the border rows next to
         // two or more clusters (hpdbscan.h:74-92)
         const int32_t namb = static_cast<int32_t>(hs[3]);
+        if (boxes && !a.pos_of_slot) {  // the relaxation compares sorted positions
+            launch_positions(n, m, box_of_slot, bcell, inrow_of_slot,
+                             reinterpret_cast<uint32_t*>(keys), vals_free,
+                             reinterpret_cast<uint32_t*>(keys_free),
+                             reinterpret_cast<uint32_t*>(scratch64), hist, tmp, pos_of_slot, s);
+            a.pos_of_slot = pos_of_slot;
+        }
         int32_t* f = A.take_n<int32_t>(nn);
         uint8_t* involved = A.take_n<uint8_t>(nn);
         unsigned int* changed = A.take_n<unsigned int>(1);
@@ -665,10 +658,15 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
     ctx.stats.key_bits = key_bits;
     HPD_HIP(hipMemcpyAsync(jobs_dev, hj.data(), sizeof(JobParams<T>) * njobs, hipMemcpyHostToDevice, s));
 
-    if (key_bits <= 32)
-        run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, labels, s, tm);
+    // floating-point data runs on sub-boxes (the sort key then carries the
+    // sub-box digit below the cell id); integer data, or a grid so large
+    // that the digit does not fit beside the cell id, runs on cells
+    const bool boxes = std::is_floating_point<T>::value && key_bits + box_sub_bits(D) <= 64;
+    const int sort_bits = key_bits + (boxes ? box_sub_bits(D) : 0);
+    if (sort_bits <= 32)
+        run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, boxes, labels, s, tm);
     else
-        run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, labels, s, tm);
+        run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, boxes, labels, s, tm);
     if (ctx.timing) {
         ctx.stats.ms_grid = tm.ms(M_START, M_GRID);
         ctx.stats.ms_sort = tm.ms(M_GRID, M_SORT);

@@ -189,10 +189,10 @@ __global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
     if (valid) core[p] = is_core ? 1 : 0;
     (void)ncore;
     if (a.bmincore) {
-        // smallest core position per box: a box inside this chunk is stored
+        // smallest core input row per box: a box inside this chunk is stored
         // by its first lane, one that extends past it is merged atomically
         const int32_t b = valid ? a.box_of_slot[p] : -1 - lane_id();
-        const int32_t m = run_suffix_min<int32_t>(is_core ? slot_pos(a.pos_of_slot, p) : INT_MAX, b);
+        const int32_t m = run_suffix_min<int32_t>(is_core ? static_cast<int32_t>(a.inrow[p]) : INT_MAX, b);
         const bool first = run_head(b);  // every lane takes part in the shuffles
         if (valid && first && m != INT_MAX) {
             if (a.bstart[b] >= c.first && a.bstart[b + 1] <= c.first + kWave)
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
             }
             is_core = cnt >= min_pts;
             core[p] = is_core ? 1 : 0;
-            if (is_core && a.bmincore) atomicMin(&a.bmincore[b], slot_pos(a.pos_of_slot, p));
+            if (is_core && a.bmincore) atomicMin(&a.bmincore[b], static_cast<int32_t>(a.inrow[p]));
         }
     }
     (void)ncore;

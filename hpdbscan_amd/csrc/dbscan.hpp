@@ -44,8 +44,8 @@ struct DbscanArgs {
     const int2* sitems = nullptr;
     int32_t nsitems = 0;
     bool sparse_core = false;  // core count split too (else one wave per chunk of every cell)
-    // box mode: the core count also writes the smallest sorted position of a
-    // core row per box (pre-filled with INT_MAX; stays INT_MAX without cores)
+    // box mode: the core count also writes the smallest input row of a core
+    // row per box (pre-filled with INT_MAX; stays INT_MAX without cores)
     int32_t* bmincore = nullptr;
 };
 

@@ -204,10 +204,28 @@ HPD_INT_CELL_INDEX(uint32_t)
 HPD_INT_CELL_INDEX(uint64_t)
 #undef HPD_INT_CELL_INDEX
 
+// (x - min) * (1/eps) as the cell index computes it (float product for
+// float data, double for double), returned exactly as a double
+template <typename T>
+__device__ __forceinline__ double scaled_value(T x, T mn, float rcp_f, double rcp_d);
+template <>
+__device__ __forceinline__ double scaled_value<float>(float x, float mn, float rcp_f, double) {
+    return static_cast<double>((x - mn) * rcp_f);
+}
+template <>
+__device__ __forceinline__ double scaled_value<double>(double x, double mn, float, double rcp_d) {
+    return (x - mn) * rcp_d;
+}
+template <typename T>
+__device__ __forceinline__ double scaled_value(T, T, float, double) {
+    return 0.0;  // integer data has no sub-boxes
+}
+
 template <typename T, int D, typename K>
 __global__ __launch_bounds__(256) void k_cell_ids(const T* __restrict__ pts, int64_t n,
                                                   const JobParams<T>* __restrict__ jobs,
-                                                  int32_t njobs, K* __restrict__ keys,
+                                                  int32_t njobs, int sub_k, int sub_bits,
+                                                  K* __restrict__ keys,
                                                   uint32_t* __restrict__ vals) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -229,7 +247,22 @@ __global__ __launch_bounds__(256) void k_cell_ids(const T* __restrict__ pts, int
             }
         cell += cell_index<T>(xj, mj, jp.rcp_f, jp.rcp_d) * jp.cp.stride[k];
     }
-    keys[i] = static_cast<K>(jp.key_base + cell);
+    // sub-box digit: the box of the cell's k^D split, along the same scaled
+    // coordinate v = (x - min) * (1/eps) that the cell index floors (v held
+    // exactly in double): sub_j = clamp(floor(v_j k) - k floor(v_j), 0, k-1)
+    uint64_t sub = 0, mul = 1;
+    if (sub_bits > 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const double v = scaled_value<T>(row[j], jp.minv[j], jp.rcp_f, jp.rcp_d);
+            const double c = floor(v);
+            long long sd = static_cast<long long>(floor(v * sub_k)) - static_cast<long long>(c) * sub_k;
+            sd = sd < 0 ? 0 : (sd >= sub_k ? sub_k - 1 : sd);
+            sub += static_cast<uint64_t>(sd) * mul;
+            mul *= static_cast<uint64_t>(sub_k);
+        }
+    }
+    keys[i] = static_cast<K>(((jp.key_base + cell) << sub_bits) | sub);
     vals[i] = static_cast<uint32_t>(i);
 }
 
@@ -261,11 +294,11 @@ void launch_minmax(const T* pts, const JobParams<T>* jobs, int32_t njobs, const 
 }
 
 template <typename T, int D, typename K>
-void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, K* keys,
-                     uint32_t* vals, hipStream_t s) {
+void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, int sub_k,
+                     int sub_bits, K* keys, uint32_t* vals, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL((k_cell_ids<T, D, K>), dim3(blocks_for(n, 256)), dim3(256), 0, s, pts, n,
-                       jobs, njobs, keys, vals);
+                       jobs, njobs, sub_k, sub_bits, keys, vals);
     HPD_HIP(hipGetLastError());
 }
 
@@ -282,9 +315,11 @@ void launch_gather_soa(const T* pts, int64_t n, const uint32_t* order, T* soa, i
     template void launch_minmax<T, D>(const T*, const JobParams<T>*, int32_t, const int32_t*,    \
                                       int32_t, T*, T*, int*, hipStream_t);                       \
     template void launch_cell_ids<T, D, uint32_t>(const T*, int64_t, const JobParams<T>*,        \
-                                                  int32_t, uint32_t*, uint32_t*, hipStream_t);   \
+                                                  int32_t, int, int, uint32_t*, uint32_t*,       \
+                                                  hipStream_t);                                  \
     template void launch_cell_ids<T, D, uint64_t>(const T*, int64_t, const JobParams<T>*,        \
-                                                  int32_t, uint64_t*, uint32_t*, hipStream_t);   \
+                                                  int32_t, int, int, uint64_t*, uint32_t*,       \
+                                                  hipStream_t);                                  \
     template void launch_gather_soa<T, D>(const T*, int64_t, const uint32_t*, T*, int64_t,      \
                                           hipStream_t);
 

@@ -35,10 +35,12 @@ template <typename T, int D>
 void launch_minmax(const T* pts, const JobParams<T>* jobs, int32_t njobs, const int32_t* bfirst,
                    int32_t nblocks, T* part, T* out, int* nonfinite, hipStream_t s);
 
-// key = job key_base + linearised cell id (spatial_index.h:144-165)
+// key = (job key_base + linearised cell id (spatial_index.h:144-165)) <<
+// sub_bits | sub-box digit (sub_bits = 0: no digit; the cell's k^D split
+// with k = sub_k), val = row
 template <typename T, int D, typename K>
-void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, K* keys,
-                     uint32_t* vals, hipStream_t s);
+void launch_cell_ids(const T* pts, int64_t n, const JobParams<T>* jobs, int32_t njobs, int sub_k,
+                     int sub_bits, K* keys, uint32_t* vals, hipStream_t s);
 
 // soa[j * ld + i] = pts[order[i] * D + j]
 template <typename T, int D>

@@ -28,14 +28,15 @@ inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n +
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_run_flags(const K* __restrict__ keys, int64_t n,
-                                                   uint32_t* __restrict__ flags) {
+                                                   int shift, uint32_t* __restrict__ flags) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
-    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+    flags[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
 }
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_compact_cells(const K* __restrict__ keys, int64_t n,
+                                                       int shift,
                                                        const uint32_t* __restrict__ flags,
                                                        const uint32_t* __restrict__ scan,
                                                        K* __restrict__ ucell,
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(256) void k_compact_cells(const K* __restrict__ key
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i < n && flags[i]) {
         const uint32_t u = scan[i];
-        ucell[u] = keys[i];
+        ucell[u] = keys[i] >> shift;
         ustart[u] = static_cast<int32_t>(i);
     }
     if (i == 0) ustart[*ncells_dev] = static_cast<int32_t>(n);
@@ -273,13 +274,15 @@ void launch_expand_chunks(const uint32_t* off, int32_t ngroups, const int32_t* s
 }
 
 template <typename K>
-void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,
-                  int32_t* ustart, uint32_t* ncells_dev, hipStream_t s) {
-    hipLaunchKernelGGL(k_run_flags<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, flags);
+void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t* scan_tmp,
+                  K* ucell, int32_t* ustart, uint32_t* ncells_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_run_flags<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, shift,
+                       flags);
     exclusive_scan_u32(flags, flags + n, n, scan_tmp, ncells_dev, s);
     hipLaunchKernelGGL(k_compact_cells<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n,
-                       static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(flags + n),
-                       ucell, ustart, static_cast<const uint32_t*>(ncells_dev));
+                       shift, static_cast<const uint32_t*>(flags),
+                       static_cast<const uint32_t*>(flags + n), ucell, ustart,
+                       static_cast<const uint32_t*>(ncells_dev));
     HPD_HIP(hipGetLastError());
 }
 
@@ -314,7 +317,7 @@ void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, i
 }
 
 #define HPD_INST_INDEX(K)                                                                     \
-    template void launch_cells<K>(const K*, int64_t, uint32_t*, uint32_t*, K*, int32_t*,      \
+    template void launch_cells<K>(const K*, int64_t, int, uint32_t*, uint32_t*, K*, int32_t*, \
                                   uint32_t*, hipStream_t);
 HPD_INST_INDEX(uint32_t)
 HPD_INST_INDEX(uint64_t)

@@ -9,11 +9,12 @@
 
 namespace hpd {
 
-// flags must hold 2*n + 1 uint32 (run flags, then their exclusive scan);
-// writes ucell[0..m), ustart[0..m] with ustart[m] = n, and m to *ncells_dev.
+// Cells of sorted keys whose cell id is key >> shift.  flags must hold
+// 2*n + 1 uint32 (run flags, then their exclusive scan); writes ucell[0..m),
+// ustart[0..m] with ustart[m] = n, and m to *ncells_dev.
 template <typename K>
-void launch_cells(const K* keys, int64_t n, uint32_t* flags, uint32_t* scan_tmp, K* ucell,
-                  int32_t* ustart, uint32_t* ncells_dev, hipStream_t s);
+void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t* scan_tmp,
+                  K* ucell, int32_t* ustart, uint32_t* ncells_dev, hipStream_t s);
 
 // job_of_cell[u] = job of cell u; nranges[u] = number of non-empty
 // neighbour rows of cell u; adds Σ_u count(u) * |candidates(u)| to *sumcand.
