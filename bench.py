@@ -23,7 +23,8 @@ The JSON line also carries
   batched       the batched workload (4096 clouds, 1k-200k rows each) in one
                 hpd_cluster_batch_device call vs a loop of one call per
                 cloud: jobs/s of both, per-call p50/p99 of the loop, and
-                equality of the labels
+                equality of the labels; the work split of both comes from
+                hpdbscan_amd.autotune run on the batch first
   serving       a Poisson stream (200 requests/s, 10k-5M rows, d in 2..4)
                 through the continuous-batching server: offered vs achieved
                 load, p50/p99 latency
@@ -44,7 +45,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from hpdbscan_amd import _lib, datagen  # noqa: E402
+from hpdbscan_amd import _lib, autotune, datagen  # noqa: E402
 
 METRIC = "Mpoints/s at 10M x 3-D (bit-exact labels); batched jobs/s + p99 vs plain"
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
@@ -171,6 +172,14 @@ def batched_leg(a) -> dict:
     print(f"[bench] batched: {nj} clouds, {len(rows)} rows (generated in {time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
     ctx = _lib.Context(0)
+    # the work split comes from the autotuner, run on this batch (every
+    # candidate must give the same labels); the plain loop uses it too
+    t0 = time.time()
+    tuned = autotune.tune_batch(rows, off, eps, mps, ctx=ctx, reps=1) if a.autotune else None
+    if tuned:
+        autotune.apply(ctx, tuned)
+        print(f"[bench] batched: work split {tuned['best']} (autotuned in {time.time() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
     xd = _lib.DeviceBuffer.from_array(ctx, rows)
     ld = _lib.DeviceBuffer(ctx, 8 * len(rows))
     # warm up both paths (allocations)
@@ -210,6 +219,8 @@ def batched_leg(a) -> dict:
         "batched_latency_ms": round(t_batch * 1e3, 2),
         "speedup_jobs_per_s": round(t_plain / t_batch, 2),
         "labels_equal": equal,
+        "work_split": tuned["best"] if tuned else dict(zip(("sparse_max", "sparse_core"),
+                                                           (8, False))),
         "note": "device-resident rows; a batched job's latency is the whole batch's",
     }
 
@@ -279,6 +290,8 @@ def main() -> None:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="batched leg: library default work split instead of the autotuned one")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)

@@ -110,6 +110,8 @@ SIGNATURES = {
     "hpd_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "hpd_context_destroy": (ctypes.c_int, [_vp]),
     "hpd_context_set_border_rule": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hpd_context_set_work_split": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int]),
+    "hpd_context_get_work_split": (ctypes.c_int, [_vp, _vp, _vp]),
     "hpd_context_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hpd_context_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(RunStats)]),
     "hpd_cluster": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
@@ -206,6 +208,16 @@ class Context:
 
     def set_border_rule(self, rule: int) -> None:
         check(load().hpd_context_set_border_rule(self._h, rule))
+
+    def set_work_split(self, sparse_max: int, sparse_core: bool = False) -> None:
+        """Cells with at most sparse_max rows go one thread per row (border
+        pass; core count too with sparse_core).  Results do not change."""
+        check(load().hpd_context_set_work_split(self._h, int(sparse_max), 1 if sparse_core else 0))
+
+    def work_split(self) -> tuple[int, bool]:
+        sm, sc = ctypes.c_int32(), ctypes.c_int()
+        check(load().hpd_context_get_work_split(self._h, ctypes.byref(sm), ctypes.byref(sc)))
+        return int(sm.value), bool(sc.value)
 
     def set_timing(self, enabled: bool) -> None:
         check(load().hpd_context_set_timing(self._h, 1 if enabled else 0))

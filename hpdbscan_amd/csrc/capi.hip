@@ -187,10 +187,11 @@ struct Context {
     Arena io;  // input rows and labels of the host-memory entry point
     int border_rule = HPD_BORDER_MIN_CLUSTER;
     // box mode: cells with at most this many rows are handled one thread
-    // per row in the core count and the border pass, the others one wave
-    // per 64-row chunk; HPD_SPARSE_MAX overrides it
+    // per row in the border pass (and in the core count if sparse_core),
+    // the others one wave per 64-row chunk (hpd_context_set_work_split;
+    // defaults picked by hpdbscan_amd.autotune on the 10M x 3-D workload)
     int32_t sparse_max = 8;
-    bool sparse_core = false;  // HPD_SPARSE_CORE=1: the core count uses the split too
+    bool sparse_core = false;
     bool timing = false;
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
@@ -798,9 +799,6 @@ int hpd_context_create(int device, hpd_context** out) {
         auto* c = new hpd_context();
         try {
             c->impl.device = device;
-            if (const char* e = std::getenv("HPD_SPARSE_MAX"))
-                c->impl.sparse_max = std::max(0, std::atoi(e));
-            if (const char* e = std::getenv("HPD_SPARSE_CORE")) c->impl.sparse_core = std::atoi(e) != 0;
             hpd::DeviceGuard g(device);
             HPD_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
             void* hs = nullptr;
@@ -842,6 +840,24 @@ int hpd_context_set_timing(hpd_context* ctx, int enabled) {
     return guarded([&] {
         if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
         ctx->impl.timing = enabled != 0;
+    });
+}
+
+int hpd_context_set_work_split(hpd_context* ctx, int32_t sparse_max, int sparse_core) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        if (sparse_max < 0 || sparse_max > 64)
+            throw Error(HPD_EINVAL, "sparse_max must be in [0, 64]");
+        ctx->impl.sparse_max = sparse_max;
+        ctx->impl.sparse_core = sparse_core != 0;
+    });
+}
+
+int hpd_context_get_work_split(const hpd_context* ctx, int32_t* sparse_max, int* sparse_core) {
+    return guarded([&] {
+        if (!ctx || !sparse_max || !sparse_core) throw Error(HPD_EINVAL, "NULL argument");
+        *sparse_max = ctx->impl.sparse_max;
+        *sparse_core = ctx->impl.sparse_core ? 1 : 0;
     });
 }
 

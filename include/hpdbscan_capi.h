@@ -114,6 +114,15 @@ int hpd_context_set_border_rule(hpd_context* ctx, int rule);
 int hpd_context_set_timing(hpd_context* ctx, int enabled);
 int hpd_context_get_stats(const hpd_context* ctx, hpd_run_stats* out);
 
+/* Work split of the neighbour kernels on sub-boxes (floating-point data):
+ * cells with at most sparse_max rows (0..64) are processed one GPU thread
+ * per row in the border pass, and in the core count too when sparse_core is
+ * nonzero; other cells one 64-lane wavefront per 64-row chunk.  Results do
+ * not depend on it; hpdbscan_amd.autotune picks it for a workload.  No
+ * counterpart in the reference (OpenMP schedule(dynamic, 32), hpdbscan.h:65). */
+int hpd_context_set_work_split(hpd_context* ctx, int32_t sparse_max, int sparse_core);
+int hpd_context_get_work_split(const hpd_context* ctx, int32_t* sparse_max, int* sparse_core);
+
 /*
  * Cluster n rows of d values of type `dtype` (row-major, host memory).
  * labels_out (host, n entries) receives the reference's labels in input
