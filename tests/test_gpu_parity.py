@@ -191,6 +191,33 @@ def test_skewed_2d_sample(ctx):
     assert_parity(x, 0.01, 32, got, oracle.cluster(x, 0.01, 32))
 
 
+def test_blobs_10m_slab_and_full_size(ctx):
+    """The headline workload: a 100k-row slab (rows with x0 below a quantile,
+    same density) against the oracle, and at the full 10M rows the
+    size-independent properties: two runs are identical, a batch of one
+    equals the plain call, every core label is negative and every cluster id
+    k has a core labelled -k."""
+    from hpdbscan_amd import _lib, datagen
+    x, eps, mp = datagen.make("blobs_10m")
+    cut = np.partition(x[:, 0], 100_000)[100_000]
+    slab = np.ascontiguousarray(x[x[:, 0] < cut])
+    say(f"blobs_10m slab n={len(slab)} vs oracle")
+    got = ctx.cluster(slab, eps, mp)
+    assert_parity(slab, eps, mp, got, oracle.cluster(slab, eps, mp))
+    say("blobs_10m full size, properties")
+    a = ctx.cluster(x, eps, mp)
+    st = ctx.stats()
+    b = ctx.cluster(x, eps, mp)
+    np.testing.assert_array_equal(a, b)
+    c = ctx.cluster_batch(x, [0, len(x)], eps, mp)
+    np.testing.assert_array_equal(a, c)
+    assert int(np.count_nonzero(a < 0)) == st["core_points"]
+    assert int(np.count_nonzero(a == 0)) == st["noise_points"]
+    ids = np.unique(np.abs(a[a != 0]))
+    core_ids = np.unique(-a[a < 0])
+    np.testing.assert_array_equal(ids, core_ids)  # every cluster has a core
+
+
 def test_batch_matches_single_jobs(ctx):
     """One batched call returns, for every job, the labels of clustering the
     job alone (and the reference's labels)."""
