@@ -40,10 +40,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
-// rounds of root joins after the hook forest: one suffices in 1-3 dims;
-// with the finer 4+-dim boxes (k = 3) a second one still halves the open pairs
-template <int D>
-constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
@@ -201,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restr
 // connected sets of boxes.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_hook(
-    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nitems,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
@@ -211,12 +207,10 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                 static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return;
-    const int2 it = bitems[wid];
-    const int32_t b = uniform(it.x);
-    if (bmincore[b] == INT_MAX) return;
-    // one hook per box is enough: its first chunk looks for it (a missed
-    // link is found by the exact union later)
-    if (uniform(it.y) != uniform(bstart[b])) return;
+    // boxes with cores; one hook per box is enough: its first chunk looks
+    // for it (a missed link is found by the exact union later)
+    const int32_t b = uniform(blist[wid]);
+    const int2 it = make_int2(b, uniform(bstart[b]));
     const int32_t end = uniform(bstart[b + 1]);
     const int32_t p = it.y + lane_id();
     const bool pcore = p < end && core[p];
@@ -259,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
 // a cluster, so far fewer pairs stay open for the union.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_root_join(
-    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nitems,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
@@ -269,10 +263,9 @@ __global__ __launch_bounds__(kThreads) void k_box_root_join(
     const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                 static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return;
-    const int2 it = bitems[wid];
-    const int32_t b = uniform(it.x);
-    if (rsnap[b] != b) return;  // not a root (or no cores)
-    if (uniform(it.y) != uniform(bstart[b])) return;  // the box's first chunk only
+    // roots of the forest, each searched from its first chunk
+    const int32_t b = uniform(blist[wid]);
+    const int2 it = make_int2(b, uniform(bstart[b]));
     const int32_t end = uniform(bstart[b + 1]);
     const int32_t p = it.y + lane_id();
     const bool pcore = p < end && core[p];
@@ -432,6 +425,44 @@ __global__ __launch_bounds__(kThreads) void k_box_union_list(
     }
 }
 
+// list[...] = every box b with (mode 0) cores / (mode 1) rsnap[b] == b (a
+// root), in no particular order; *count += their number
+__global__ __launch_bounds__(kThreads) void k_select_boxes(int32_t nbox, int mode,
+                                                           const int32_t* __restrict__ v,
+                                                           int32_t* __restrict__ list,
+                                                           unsigned int* __restrict__ count) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool take = false;
+    if (b < nbox) take = mode == 0 ? v[b] != INT_MAX : v[b] == b;
+    const uint64_t mask = ballot(take);
+    if (!mask) return;
+    unsigned int base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned int>(__popcll(mask)));
+    base = __shfl(base, 0, 64);
+    if (take) list[base + __popcll(mask & lanemask_lt())] = b;
+}
+
+// the 64-slot chunks of the boxes that have open pairs (off[b+1] > off[b])
+__global__ __launch_bounds__(kThreads) void k_select_items(int32_t nitems,
+                                                           const int2* __restrict__ items,
+                                                           const uint32_t* __restrict__ off,
+                                                           int2* __restrict__ out,
+                                                           unsigned int* __restrict__ count) {
+    const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool take = false;
+    int2 it = make_int2(0, 0);
+    if (i < nitems) {
+        it = items[i];
+        take = off[it.x + 1] > off[it.x];
+    }
+    const uint64_t mask = ballot(take);
+    if (!mask) return;
+    unsigned int base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned int>(__popcll(mask)));
+    base = __shfl(base, 0, 64);
+    if (take) out[base + __popcll(mask & lanemask_lt())] = it;
+}
+
 // root of every box's hook tree -> rsnap (-1 for boxes without cores)
 __global__ __launch_bounds__(kThreads) void k_box_roots(int32_t nbox,
                                                         const int32_t* __restrict__ bmincore,
@@ -555,26 +586,23 @@ __device__ __forceinline__ int32_t count_below(const uint32_t* __restrict__ v, i
     return lo - first;
 }
 
-// position of every component root's smallest core (one wave per box;
-// non-roots leave at once)
+// position of every component root's smallest core, one thread per box
+// (non-roots leave at once)
 __global__ __launch_bounds__(kThreads) void k_box_resolve(
     int32_t nbox, const int32_t* __restrict__ bparent, const unsigned long long* __restrict__ cmin,
     const int32_t* __restrict__ cbox, const int32_t* __restrict__ bstart,
     const uint32_t* __restrict__ inrow, const int32_t* __restrict__ ustart,
     int32_t* __restrict__ pos) {
-    const int32_t r = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                              static_cast<int32_t>(threadIdx.x >> 6));
+    const int32_t r = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (r >= nbox || bparent[r] != r) return;
     const unsigned long long key = cmin[r];
     if (key == ~0ull) return;  // no cores
     const int32_t u = static_cast<int32_t>(key >> 32);
     const uint32_t row = static_cast<uint32_t>(key & 0xffffffffu);
     int32_t below = 0;
-    for (int32_t b2 = uniform(cbox[u]) + lane_id(); b2 < cbox[u + 1]; b2 += kWave)
+    for (int32_t b2 = cbox[u]; b2 < cbox[u + 1]; ++b2)
         below += count_below(inrow, bstart[b2], bstart[b2 + 1], row);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);
-    if (lane_id() == 0) pos[r] = ustart[u] + below;
+    pos[r] = ustart[u] + below;
 }
 
 // cluster id (smallest core position of the component) per box with cores
@@ -686,27 +714,53 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
 
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                     const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
-                     int32_t* rsnap, hipStream_t s) {
+                     const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
+                     int32_t nlist, int32_t* bparent, hipStream_t s) {
     using U = OrdKey<T>;
     hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        bparent, static_cast<int64_t>(a.nbox));
-    hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
-                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
-                       a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
+    if (nlist > 0)
+        hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(nlist, kWavesPerBlock)),
+                           dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell,
+                           a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
+                           a.job_of_cell, a.jobs, a.rcenter, bparent);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_box_roots(int32_t nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
+                      hipStream_t s) {
+    if (nbox <= 0) return;
+    hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, nbox,
+                       bmincore, bparent, rsnap);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                          const int32_t* rsnap, const uint8_t* core, const int32_t* blist,
+                          int32_t nlist, int32_t* bparent, hipStream_t s) {
+    using U = OrdKey<T>;
+    if (nlist <= 0) return;
+    hipLaunchKernelGGL((k_box_root_join<T, D>), dim3(blocks_for(nlist, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell, a.cbox,
+                       roff, rcells, reinterpret_cast<const U*>(a.bbox), rsnap, core,
                        a.job_of_cell, a.jobs, a.rcenter, bparent);
-    hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                       a.nbox, bmincore, bparent, rsnap);
-    // rounds of root joins, each followed by the snapshot again
-    for (int round = 0; round < kRootJoinRounds<D>; ++round) {
-        hipLaunchKernelGGL((k_box_root_join<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart,
-                           a.bcell, a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox),
-                           static_cast<const int32_t*>(rsnap), core, a.job_of_cell, a.jobs,
-                           a.rcenter, bparent);
-        hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                           a.nbox, bmincore, bparent, rsnap);
-    }
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list,
+                         unsigned int* count, hipStream_t s) {
+    if (nbox <= 0) return;
+    hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+                       nbox, mode, v, list, count);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
+                         unsigned int* count, hipStream_t s) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems, kThreads)), dim3(kThreads), 0, s,
+                       nitems, items, off, out, count);
     HPD_HIP(hipGetLastError());
 }
 
@@ -734,11 +788,13 @@ void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
 }
 
 template <typename T, int D>
-void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint32_t* off,
-                           const int32_t* list, int32_t* bparent, hipStream_t s) {
+void launch_box_union_list(const BoxArgs<T>& a, const int2* items, int32_t nitems,
+                           const uint8_t* core, const uint32_t* off, const int32_t* list,
+                           int32_t* bparent, hipStream_t s) {
     using U = OrdKey<T>;
-    hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
-                       dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(blocks_for(nitems, kWavesPerBlock)),
+                       dim3(kThreads), 0, s, a.soa, a.ld, items, nitems, a.bstart, a.bcell,
                        reinterpret_cast<const U*>(a.bbox), core, a.job_of_cell, a.jobs, off, list,
                        bparent);
     HPD_HIP(hipGetLastError());
@@ -759,7 +815,7 @@ void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* b
                        ~0ull, static_cast<int64_t>(nbox));
     hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
                        nbox, bmincore, bcell, bparent, cmin);
-    hipLaunchKernelGGL(k_box_resolve, dim3(blocks_for(nbox, kWavesPerBlock)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_box_resolve, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
                        nbox, static_cast<const int32_t*>(bparent),
                        static_cast<const unsigned long long*>(cmin), cbox, bstart, inrow, ustart,
                        compmin);
@@ -810,16 +866,20 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
-                                        const int32_t*, const uint8_t*, int32_t*, int32_t*,       \
-                                        hipStream_t);                                             \
+                                        const int32_t*, const uint8_t*, const int32_t*, int32_t,  \
+                                        int32_t*, hipStream_t);                                   \
+    template void launch_box_root_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,     \
+                                             const int32_t*, const uint8_t*, const int32_t*,      \
+                                             int32_t, int32_t*, hipStream_t);                     \
     template void launch_box_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, const uint8_t*, const int32_t*, int32_t*, \
                                         hipStream_t);                                             \
     template void launch_box_scan<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, uint32_t*, int32_t*,                      \
                                         unsigned long long*, hipStream_t);                        \
-    template void launch_box_union_list<T, D>(const BoxArgs<T>&, const uint8_t*, const uint32_t*, \
-                                              const int32_t*, int32_t*, hipStream_t);
+    template void launch_box_union_list<T, D>(const BoxArgs<T>&, const int2*, int32_t,             \
+                                              const uint8_t*, const uint32_t*, const int32_t*,    \
+                                              int32_t*, hipStream_t);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \
     HPD_INST_BOX_KEYS(T, 2)   \

@@ -61,23 +61,40 @@ void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* 
 template <typename T, int D>
 void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s);
 
-// Components of the boxes with cores, in three launches:
-//   launch_box_hook      hook forest (every box points at some earlier box
-//                        joined to it) and its roots -> rsnap (nbox entries,
-//                        -1 for boxes without cores)
-//   launch_box_join      union of the box pairs the snapshot does not join
-//   launch_box_clusters  bcluster[b] = smallest core position of b's
-//                        component (-1 without cores); bmincore holds the
-//                        smallest core input row per box; uses cmin and
-//                        compmin (nbox each) as scratch
+// Components of the boxes with cores:
+//   launch_select_boxes  list of the boxes with cores (mode 0, v = bmincore)
+//                        or of the roots (mode 1, v = rsnap); *count += n
+//   launch_box_hook      hook forest over the listed boxes (bparent is
+//                        reset first): every box points at some earlier box
+//                        joined to it
+//   launch_box_roots     root of every box -> rsnap (-1 without cores)
+//   launch_box_root_join every listed root joins the first tree of another
+//                        box it meets (kRootJoinRounds rounds, each followed
+//                        by launch_box_roots)
+//   then the open pairs (below), launch_box_clusters.
+template <int D>
+constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
+
+void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list,
+                         unsigned int* count, hipStream_t s);
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
-                     const int32_t* bmincore, const uint8_t* core, int32_t* bparent,
-                     int32_t* rsnap, hipStream_t s);
+                     const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
+                     int32_t nlist, int32_t* bparent, hipStream_t s);
+void launch_box_roots(int32_t nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
+                      hipStream_t s);
+template <typename T, int D>
+void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
+                          const int32_t* rsnap, const uint8_t* core, const int32_t* blist,
+                          int32_t nlist, int32_t* bparent, hipStream_t s);
+// the box chunks (items) of the boxes with open pairs (off[b+1] > off[b])
+void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
+                         unsigned int* count, hipStream_t s);
 template <typename T, int D>
 void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
                      int32_t* bparent, hipStream_t s);
+
 // The pairs the hook forest leaves open.  launch_box_scan with list ==
 // nullptr writes cnt[b] (nbox entries) and adds their sum to *total (64-bit,
 // zeroed by the caller); with a list it writes every box's open partners to
@@ -90,9 +107,9 @@ void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
                      const int32_t* rsnap, uint32_t* cnt, int32_t* list,
                      unsigned long long* total, hipStream_t s);
 template <typename T, int D>
-void launch_box_union_list(const BoxArgs<T>& a, const uint8_t* core, const uint32_t* off,
-                           const int32_t* list, int32_t* bparent, hipStream_t s);
-// p[0, n) = v
+void launch_box_union_list(const BoxArgs<T>& a, const int2* items, int32_t nitems,
+                           const uint8_t* core, const uint32_t* off, const int32_t* list,
+                           int32_t* bparent, hipStream_t s);// p[0, n) = v
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 
 void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* bcell,

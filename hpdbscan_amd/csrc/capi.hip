@@ -414,13 +414,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                     align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nditems) * sizeof(int2), 256) +
-                    align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) + 2048);
+                    align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) + 2048);
     int2* ranges = ctx.aux.take_n<int2>(total_ranges);
     int2* rcells = ctx.aux.take_n<int2>(total_ranges);
     int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
     int2* bitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
     int2* ditems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nditems)) : nullptr;
     int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
+    int2* uitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
     launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
                                rcenter, s);
     launch_work_items(ustart, m, nchunks, nitems, items, s);
@@ -481,30 +483,50 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     launch_core<T, D>(a, core, &dstats[1], s);
     tm.mark(M_CORE);
 
+    // box lists of the union (box mode leaves the slot-level parent free)
+    int32_t* blist = parent;
+    int32_t nlist = 0;
     if (boxes) {
-        HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        launch_select_boxes(nbox, 0, bmincore, blist, &counters[8], s);
+        HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HPD_HIP(hipStreamSynchronize(s));
         box_union = hc[5] == 0;  // every box of two or more rows is tight
+        nlist = static_cast<int32_t>(hc[8]);  // boxes with cores
     }
     if (box_union) {
-        // compmin holds the hook forest's roots until the clusters are known
+        // compmin holds the forest's roots until the clusters are known
         tm.mark(M_HOOK0);
-        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, bparent, compmin, s);
+        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, nlist, bparent, s);
+        launch_box_roots(nbox, bmincore, bparent, compmin, s);
+        for (int round = 0; round < kRootJoinRounds<D>; ++round) {
+            HPD_HIP(hipMemsetAsync(&counters[9], 0, sizeof(uint32_t), s));
+            launch_select_boxes(nbox, 1, compmin, blist, &counters[9], s);
+            HPD_HIP(hipMemcpyAsync(hc + 9, &counters[9], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HPD_HIP(hipStreamSynchronize(s));
+            launch_box_root_join<T, D>(ba, nranges, rcells, compmin, core, blist,
+                                       static_cast<int32_t>(hc[9]), bparent, s);
+            launch_box_roots(nbox, bmincore, bparent, compmin, s);
+        }
         tm.mark(M_HOOK1);
         // the open pairs of every box, as a list in scratch64; offsets in
         // bcnt (free after the box items)
         int32_t* list = reinterpret_cast<int32_t*>(scratch64);
         const uint64_t cap = 2 * static_cast<uint64_t>(nn);
         launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
+        // offsets (meaningful when the total fits, checked below) and the
+        // chunks of the boxes that have open pairs
+        exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
+        launch_select_items(nbitems, bitems, bcnt, uitems, &counters[10], s);
         unsigned long long* hs64 = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
         HPD_HIP(hipMemcpyAsync(hs64 + 8, &dstats[5], sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, s));
+        HPD_HIP(hipMemcpyAsync(hc + 20, &counters[10], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HPD_HIP(hipStreamSynchronize(s));
         ctx.stats.open_box_pairs = hs64[8];
         if (hs64[8] <= cap) {  // then the 32-bit offsets cannot overflow
-            exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
             launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s);
-            launch_box_union_list<T, D>(ba, core, bcnt, list, bparent, s);
+            launch_box_union_list<T, D>(ba, uitems, static_cast<int32_t>(hc[20]), core, bcnt, list,
+                                        bparent, s);
         } else {
             launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
         }
