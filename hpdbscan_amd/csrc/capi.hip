@@ -274,7 +274,7 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     b += 4 * hist_words + 4 * tmp_words;       // radix histogram, scan scratch
     b += sizeof(T) * L * D;                    // coordinates in slot order
     b += 4 * (2 * nn + 2);                     // run flags + scan
-    b += 8 * nn + 3 * 4 * (nn + 1);            // ucell, ustart, nranges, nchunks
+    b += 8 * nn + 4 * 4 * (nn + 1);            // ucell, ustart, nranges, nchunks, core chunks
     b += L + 3 * 4 * L + 3 * 4 * nn;           // core, parent, rsnap, rootslot, cmin/cmax/rcenter
     b += 8 * nn + 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // scratch64, positions, box tables
     b += 2 * D * 8 * nn;                       // box bounding boxes
@@ -378,7 +378,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     tm.mark(M_SORT);
 
     // neighbour ranges (spatial_index.h:682-737) and work items
-    launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, &dstats[0], s);
+    uint32_t* ccnt = boxes ? A.take_n<uint32_t>(nn + 1) : nullptr;
+    launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
+                                &dstats[0], s);
     exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
     exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
@@ -387,6 +389,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     uint32_t* dcnt = flags;
     uint32_t* scnt = flags + nn + 1;
     if (boxes) {
+        exclusive_scan_u32(ccnt, ccnt, m, tmp, &ccnt[m], s);
+        HPD_HIP(hipMemcpyAsync(&counters[11], &ccnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         launch_split_counts(ustart, m, ctx.sparse_max, dcnt, scnt, s);
         exclusive_scan_u32(dcnt, dcnt, m, tmp, &dcnt[m], s);
         exclusive_scan_u32(scnt, scnt, m, tmp, &scnt[m], s);
@@ -395,9 +399,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     }
     HPD_HIP(hipMemcpyAsync(&counters[1], &nranges[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(&counters[2], &nchunks[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    HPD_HIP(hipMemcpyAsync(hc, counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hc, counters, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
     const uint32_t total_ranges = hc[1];
+    const int32_t ncitems = boxes ? static_cast<int32_t>(hc[11]) : 0;
     const int32_t nitems = static_cast<int32_t>(hc[2]);
     const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
     const int32_t nditems = boxes ? static_cast<int32_t>(hc[6]) : 0;
@@ -415,7 +420,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                     align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nditems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) +
-                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) + 2048);
+                    align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
+                    align_up(static_cast<size_t>(ncitems) * sizeof(int2), 256) + 2048);
     int2* ranges = ctx.aux.take_n<int2>(total_ranges);
     int2* rcells = ctx.aux.take_n<int2>(total_ranges);
     int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
@@ -423,11 +429,14 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* ditems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nditems)) : nullptr;
     int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
     int2* uitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
+    int2* citems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
     launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
                                rcenter, s);
     launch_work_items(ustart, m, nchunks, nitems, items, s);
-    if (boxes)
+    if (boxes) {
         launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, nditems, sitems, s);
+        launch_expand_chunks(ccnt, m, ustart, ncitems, citems, s);
+    }
 
     DbscanArgs<T> a;
     a.soa = soa;
@@ -475,6 +484,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         fill_i32(bmincore, INT_MAX, nbox, s);
         a.ditems = ditems;
         a.nditems = nditems;
+        a.citems = citems;
+        a.ncitems = ncitems;
         a.sitems = sitems;
         a.nsitems = nsitems;
     }

@@ -830,11 +830,13 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
     if constexpr (std::is_floating_point<T>::value) {
         if (a.bbox) {
             DbscanArgs<T> ac = a;
-            if (!a.sparse_core) {  // every cell's chunks as waves
-                ac.ditems = a.items;
-                ac.nditems = a.nitems;
+            if (!a.sparse_core) {  // the chunks of every cell that can hold cores
+                ac.ditems = a.citems ? a.citems : a.items;
+                ac.nditems = a.citems ? a.ncitems : a.nitems;
                 ac.nsitems = 0;
             }
+            if (a.citems && !a.sparse_core)  // rows of the other cells are not core
+                HPD_HIP(hipMemsetAsync(core, 0, static_cast<size_t>(a.n), s));
             if (ac.nditems > 0)
                 hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(ac.nditems, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, ac, core, ncore);

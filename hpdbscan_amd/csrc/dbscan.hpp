@@ -44,6 +44,10 @@ struct DbscanArgs {
     const int2* sitems = nullptr;
     int32_t nsitems = 0;
     bool sparse_core = false;  // core count split too (else one wave per chunk of every cell)
+    // chunks of the cells with >= minPts candidates: the only cells that can
+    // hold cores (core flags of the others stay 0; null: every chunk)
+    const int2* citems = nullptr;
+    int32_t ncitems = 0;
     // box mode: the core count also writes the smallest input row of a core
     // row per box (pre-filled with INT_MAX; stays INT_MAX without cores)
     int32_t* bmincore = nullptr;

@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              int32_t njobs,
                                                              int32_t* __restrict__ job_of_cell,
                                                              uint32_t* __restrict__ nranges,
+                                                             uint32_t* __restrict__ core_chunks,
                                                              unsigned long long* __restrict__ sumcand) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
@@ -153,6 +154,11 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         });
         nranges[u] = cnt;
         count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
+        // the reference queries a row only with >= minPts candidates
+        // (hpdbscan.h:74); with fewer, no row of the cell can be core
+        if (core_chunks)
+            core_chunks[u] = cand >= jobs[j].min_pts ? static_cast<uint32_t>((count_u + kWave - 1) / kWave)
+                                                     : 0u;
     }
     // Σ over rows of the cell of |candidates| = count_u * cand
     unsigned long long v = static_cast<unsigned long long>(cand * count_u);
@@ -289,9 +295,10 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
-                           uint32_t* nranges, unsigned long long* sumcand, hipStream_t s) {
+                           uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
+                           hipStream_t s) {
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
-                       ucell, ustart, m, jobs, njobs, job_of_cell, nranges, sumcand);
+                       ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand);
     HPD_HIP(hipGetLastError());
 }
 
@@ -325,11 +332,11 @@ HPD_INST_INDEX(uint64_t)
 #define HPD_INST_ROWS(T)                                                                        \
     template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
-                                                     uint32_t*, unsigned long long*,            \
+                                                     uint32_t*, uint32_t*, unsigned long long*, \
                                                      hipStream_t);                              \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
-                                                     uint32_t*, unsigned long long*,            \
+                                                     uint32_t*, uint32_t*, unsigned long long*, \
                                                      hipStream_t);                              \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
                                                     const JobParams<T>*, const int32_t*,        \

@@ -17,11 +17,14 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
                   K* ucell, int32_t* ustart, uint32_t* ncells_dev, hipStream_t s);
 
 // job_of_cell[u] = job of cell u; nranges[u] = number of non-empty
-// neighbour rows of cell u; adds Σ_u count(u) * |candidates(u)| to *sumcand.
+// neighbour rows of cell u; core_chunks[u] (when not null) = 64-row chunks
+// of cell u if it has >= minPts candidates, else 0; adds
+// Σ_u count(u) * |candidates(u)| to *sumcand.
 template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
-                           uint32_t* nranges, unsigned long long* sumcand, hipStream_t s);
+                           uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
+                           hipStream_t s);
 
 // ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row,
 // ascending; rcells[..] = [first, end) occupied-cell index of the same row;
