@@ -54,14 +54,26 @@ __global__ __launch_bounds__(256) void k_compact_cells(const K* __restrict__ key
 // first index u in [0, m) with ucell[u] >= key (m if none)
 template <typename K>
 __device__ __forceinline__ int32_t lower_bound_cells(const K* __restrict__ ucell, int32_t m,
-                                                     uint64_t key) {
-    int32_t lo = 0, hi = m;
+                                                     uint64_t key, int32_t lo = 0) {
+    int32_t hi = m;
     while (lo < hi) {
         const int32_t mid = lo + ((hi - lo) >> 1);
         if (static_cast<uint64_t>(ucell[mid]) < key) lo = mid + 1;
         else hi = mid;
     }
     return lo;
+}
+
+// Occupied cells [a, b) with ids in [lo, hi] (at most three consecutive
+// ids): a by binary search from `from` (rows are visited in ascending
+// order, so the previous row's end is a lower bound), b by a short scan.
+template <typename K>
+__device__ __forceinline__ int2 row_cells(const K* __restrict__ ucell, int32_t m, uint64_t lo,
+                                          uint64_t hi, int32_t from) {
+    const int32_t a = lower_bound_cells(ucell, m, lo, from);
+    int32_t b = a;
+    while (b < m && static_cast<uint64_t>(ucell[b]) <= hi) ++b;
+    return make_int2(a, b);
 }
 
 // Decode a cell id into per-swapped-dimension indices.
@@ -144,12 +156,13 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         job_of_cell[u] = j;
         const uint64_t base = jobs[j].key_base;
         uint32_t cnt = 0;
+        int32_t from = 0;
         for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool) {
-            const int32_t a = lower_bound_cells(ucell, m, base + lo);
-            const int32_t b = lower_bound_cells(ucell, m, base + hi + 1);
-            if (b > a) {
+            const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
+            from = ab.y;
+            if (ab.y > ab.x) {
                 ++cnt;
-                cand += static_cast<uint64_t>(ustart[b] - ustart[a]);
+                cand += static_cast<uint64_t>(ustart[ab.y] - ustart[ab.x]);
             }
         });
         nranges[u] = cnt;
@@ -183,14 +196,15 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
     const uint64_t base = jobs[j].key_base;
     uint32_t w = roff[u];
     int32_t center = static_cast<int32_t>(w);
+    int32_t from = 0;
     for_each_row(static_cast<uint64_t>(ucell[u]) - base, jobs[j].cp,
                  [&](uint64_t lo, uint64_t hi, bool is_center) {
-                     const int32_t a = lower_bound_cells(ucell, m, base + lo);
-                     const int32_t b = lower_bound_cells(ucell, m, base + hi + 1);
-                     if (b > a) {
+                     const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
+                     from = ab.y;
+                     if (ab.y > ab.x) {
                          if (is_center) center = static_cast<int32_t>(w);
-                         rcells[w] = make_int2(a, b);
-                         ranges[w++] = make_int2(ustart[a], ustart[b]);
+                         rcells[w] = ab;
+                         ranges[w++] = make_int2(ustart[ab.x], ustart[ab.y]);
                      }
                  });
     rcenter[u] = center;
