@@ -40,6 +40,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
+// boxes of at most kSmallBox rows are hooked one thread per box, with at
+// most kSmallTests distance tests per box pair
+constexpr int kSmallBox = 8;
+constexpr int kSmallTests = 512;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
@@ -211,6 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     // for it (a missed link is found by the exact union later)
     const int32_t b = uniform(blist[wid]);
     const int2 it = make_int2(b, uniform(bstart[b]));
+    if (uniform(bstart[b + 1]) - it.y <= kSmallBox) return;  // k_box_hook_small
     const int32_t end = uniform(bstart[b + 1]);
     const int32_t p = it.y + lane_id();
     const bool pcore = p < end && core[p];
@@ -420,6 +425,65 @@ __global__ __launch_bounds__(kThreads) void k_box_union_list(
                                   stop)) {
                 if (lane_id() == 0) uf_unite(bparent, rb, rb2);
                 rb = uniform(uf_find(bparent, b));
+            }
+        }
+    }
+}
+
+// Hook of a small box (at most kSmallBox rows), one thread per box: the
+// box's core rows meet the earlier boxes of its neighbourhood, own neighbour
+// row first; a box pair gets at most kSmallTests distance tests (rows
+// farther than eps from the other box's bounding box take none) before the
+// search moves on: a missed hook only leaves work for the exact union.
+
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_hook_small(
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nlist,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
+    const int2* __restrict__ rcells, const int32_t* __restrict__ rcenter,
+    const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ bmincore,
+    const uint8_t* __restrict__ core, const int32_t* __restrict__ job_of_cell,
+    const JobParams<T>* __restrict__ jobs, int32_t* __restrict__ bparent) {
+    const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= nlist) return;
+    const int32_t b = blist[i];
+    const int32_t s0 = bstart[b], s1 = bstart[b + 1];
+    if (s1 - s0 > kSmallBox) return;  // one wave per box instead
+    const int32_t u = bcell[b];
+    const JobParams<T>& jp = jobs[job_of_cell[u]];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    const int32_t r0 = static_cast<int32_t>(roff[u]), r1 = static_cast<int32_t>(roff[u + 1]);
+    const int32_t center = rcenter[u];
+    for (int32_t k = -1; k < r1 - r0; ++k) {
+        const int32_t r = k < 0 ? center : r0 + k;
+        if (k >= 0 && r == center) continue;
+        const int2 cs = rcells[r];
+        const int32_t e0 = cbox[cs.x];
+        if (e0 >= b) continue;  // only boxes before b
+        const int32_t e1 = min(cbox[cs.y], b);
+        for (int32_t b2 = e0; b2 < e1; ++b2) {
+            if (bmincore[b2] == INT_MAX) continue;
+            const int32_t t0 = bstart[b2], t1 = bstart[b2 + 1];
+            int tests = 0;
+            for (int32_t p = s0; p < s1 && tests < kSmallTests; ++p) {
+                if (!core[p]) continue;
+                T pc[D];
+                load_row<T, D>(soa, ld, p, pc);
+                double gmin2, gmax2;
+                point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+                if (gmin2 > gap2_max) continue;
+                for (int32_t q = t0; q < t1 && tests < kSmallTests; ++q) {
+                    if (!core[q]) continue;
+                    ++tests;
+                    T qc[D];
+                    load_row<T, D>(soa, ld, q, qc);
+                    if (within<D>(pc, qc, eps2)) {
+                        bparent[b] = b2;
+                        return;
+                    }
+                }
             }
         }
     }
@@ -719,11 +783,16 @@ void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
     using U = OrdKey<T>;
     hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                        bparent, static_cast<int64_t>(a.nbox));
-    if (nlist > 0)
+    if (nlist > 0) {
         hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(nlist, kWavesPerBlock)),
                            dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell,
                            a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
                            a.job_of_cell, a.jobs, a.rcenter, bparent);
+        hipLaunchKernelGGL((k_box_hook_small<T, D>), dim3(blocks_for(nlist, kThreads)),
+                           dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell,
+                           a.cbox, roff, rcells, a.rcenter, reinterpret_cast<const U*>(a.bbox),
+                           bmincore, core, a.job_of_cell, a.jobs, bparent);
+    }
     HPD_HIP(hipGetLastError());
 }
 
