@@ -42,7 +42,7 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 // boxes of at most kSmallBox rows are hooked one thread per box, with at
 // most kSmallTests distance tests per box pair
-constexpr int kSmallBox = 8;
+constexpr int kSmallBox = 32;
 constexpr int kSmallTests = 512;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
