@@ -16,8 +16,11 @@ Behaviour mirrored from the binding:
   ``cluster<double>`` (hpdbscan.h:484).  A non-2-D input raises ``TypeError``.
   The result is a tuple of ints (the ``ClusterVector`` of hpdbscan.i:20).
 * ``cluster(path, dataset[, threads])`` -- the HDF5 entry point
-  (hpdbscan.h:300-304).  HDF5 input is not part of this build and raises
-  ``NotImplementedError``.
+  (hpdbscan.h:300-356): the 2-D dataset is clustered in its stored element
+  type (no float64 conversion, as in the reference, where only the array
+  overload goes through numpy.i); ``threads`` must be an int and is
+  otherwise unused.  A missing file or dataset, or a dataset that is not a
+  matrix, raises ``ValueError`` with the reference's message (io.h:49-66).
 
 Extensions (not in the reference): ``cluster(data, as_array=True)`` returns
 an ``int64`` ndarray instead of a tuple, and ``cluster_native(data)`` keeps the
@@ -30,7 +33,7 @@ import numbers
 
 import numpy as np
 
-from . import _lib
+from . import _lib, h5
 
 __all__ = ["HPDBSCAN"]
 
@@ -94,7 +97,11 @@ class HPDBSCAN:
         if len(args) in (2, 3) and all(isinstance(x, str) for x in args[:2]):
             if len(args) == 3 and (isinstance(args[2], bool) or not isinstance(args[2], numbers.Integral)):
                 raise TypeError("in method 'HPDBSCAN_cluster', argument 4 of type 'int'")
-            raise NotImplementedError("HDF5 input (cluster(path, dataset)) is not part of this build")
+            # the file is read before the device is touched, as in the
+            # reference, so file errors come first
+            h5.matrix_info(args[0], args[1])
+            labels = h5.cluster_file(args[0], args[1], self._epsilon, self._min_points, self._ctx())
+            return labels if as_array else tuple(labels.tolist())
         raise TypeError("Wrong number or type of arguments for overloaded function 'HPDBSCAN_cluster'")
 
     def cluster_native(self, data) -> np.ndarray:

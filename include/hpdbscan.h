@@ -15,8 +15,12 @@
  * for source compatibility; the GPU path does not use host threads.
  * The input is read, never modified, as in the reference (dataset.h:40-42).
  *
- * Differences: the HDF5 overloads throw std::runtime_error (HDF5 I/O is not
- * part of this build); a NaN epsilon or non-finite coordinates throw
+ * The HDF5 overloads read the 2-D dataset in its stored element type
+ * (hpdbscan.h:306-355) through libhpdbscan_amd_h5.so (hpdbscan_h5.h): link
+ * with -lhpdbscan_amd_h5 when calling them.  IO::write_hdf5 (io.h:113-194)
+ * is hpdbscan_amd::write_hdf5 below.
+ *
+ * Differences: a NaN epsilon or non-finite coordinates throw
  * std::invalid_argument instead of producing undefined grid sizes.
  * One HPDBSCAN object owns one device context; copies share it and must not
  * be used from two threads at once.
@@ -32,6 +36,7 @@
 #include <vector>
 
 #include "hpdbscan_capi.h"
+#include "hpdbscan_h5.h"
 
 using Cluster = ptrdiff_t;
 using Clusters = std::vector<Cluster>;
@@ -56,6 +61,25 @@ inline void throw_status(int status) {
     const std::string msg = hpd_last_error();
     if (status == HPD_EINVAL) throw std::invalid_argument(msg);
     throw std::runtime_error(msg);
+}
+
+// Status of an hpd_h5_* / hpd_cluster_h5 call as the reference's exceptions.
+inline void throw_h5_status(int status) {
+    if (status == HPD_OK) return;
+    const std::string msg = hpd_h5_last_error();
+    if (status == HPD_EINVAL) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// IO::write_hdf5 (io.h:113-194): the labels as a 1-D native-long dataset of
+// `path`, created if missing; an existing dataset of that name is written in
+// place.
+inline void write_hdf5(const std::string& path, const std::string& dataset,
+                       const Clusters& clusters) {
+    static_assert(sizeof(Cluster) == sizeof(int64_t), "ptrdiff_t must be 64-bit");
+    throw_h5_status(hpd_h5_write_clusters(path.c_str(), dataset.c_str(),
+                                          reinterpret_cast<const int64_t*>(clusters.data()),
+                                          static_cast<int64_t>(clusters.size())));
 }
 
 }  // namespace hpdbscan_amd
@@ -86,10 +110,17 @@ public:
     }
 
     Clusters cluster(const std::string& path, const std::string& dataset, int threads) {
-        (void)path;
-        (void)dataset;
         (void)threads;
-        throw std::runtime_error("HDF5 input (cluster(path, dataset)) is not part of this build");
+        int64_t rows = 0;
+        int32_t cols = 0, dtype = 0;
+        hpdbscan_amd::throw_h5_status(
+            hpd_h5_matrix_info(path.c_str(), dataset.c_str(), &rows, &cols, &dtype));
+        Clusters out(static_cast<size_t>(rows));
+        static_assert(sizeof(Cluster) == sizeof(int64_t), "ptrdiff_t must be 64-bit");
+        hpdbscan_amd::throw_h5_status(hpd_cluster_h5(context(), path.c_str(), dataset.c_str(),
+                                                     m_epsilon, static_cast<int64_t>(m_min_points),
+                                                     reinterpret_cast<int64_t*>(out.data()), rows));
+        return out;
     }
 
     template <typename T>

@@ -32,7 +32,9 @@ typedef enum hpd_status {
     HPD_ENOMEM = 2,      /* device or host allocation failed */
     HPD_EDEVICE = 3,     /* HIP runtime error or no usable device */
     HPD_EUNSUPPORTED = 4, /* valid input this build does not handle */
-    HPD_ETIMEOUT = 5      /* hpd_server_wait: the request is not finished yet */
+    HPD_ETIMEOUT = 5,     /* hpd_server_wait: the request is not finished yet */
+    HPD_EIO = 6           /* file I/O failed (libhpdbscan_amd_h5.so); maps to
+                           * std::runtime_error */
 } hpd_status;
 
 /* Element types of the input rows; the reference instantiates the same ten
