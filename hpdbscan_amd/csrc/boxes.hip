@@ -348,10 +348,7 @@ __global__ __launch_bounds__(kThreads) void k_box_scan(
     if (!list) {
         if (b < nbox) cnt[b] = k;
         // 64-bit total, so that the caller can check it before a 32-bit scan
-        unsigned long long w = k;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
-        if (lane_id() == 0 && w) atomicAdd(total, w);
+        block_add<kThreads>(k, total);
     }
 }
 
@@ -491,19 +488,62 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
 
 // list[...] = every box b with (mode 0) cores / (mode 1) rsnap[b] == b (a
 // root), in no particular order; *count += their number
+// Stream compaction with one atomic per block: block `blockIdx.x` takes
+// the kSelectPer * kThreads consecutive indices from blockIdx.x *
+// kSelectSpan, wave w the 64 * kSelectPer of them from w * 64 * kSelectPer,
+// in rounds of 64.  The waves' counts meet in LDS, thread 0 reserves the
+// block's range with one atomicAdd, and every taken index i is emitted at
+// its place in the range (input order inside a block; blocks in the order
+// of their reservations).  One reservation per wave instead serialises
+// thousands of atomics on one address.
+constexpr int kSelectPer = 8;
+constexpr int kSelectSpan = kThreads * kSelectPer;
+
+template <typename Take, typename Emit>
+__device__ __forceinline__ void block_select(int64_t n, unsigned int* count, Take take,
+                                             Emit emit) {
+    __shared__ unsigned int wcount[kWavesPerBlock];
+    __shared__ unsigned int bbase;
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kSelectSpan +
+                       static_cast<int64_t>(w) * kWave * kSelectPer + lane_id();
+    uint64_t masks[kSelectPer];
+    unsigned int total = 0;
+#pragma unroll
+    for (int r = 0; r < kSelectPer; ++r) {
+        const int64_t i = i0 + r * kWave;
+        masks[r] = ballot(i < n && take(i));
+        total += static_cast<unsigned int>(__popcll(masks[r]));
+    }
+    if (lane_id() == 0) wcount[w] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+#pragma unroll
+        for (int k = 0; k < kWavesPerBlock; ++k) t += wcount[k];
+        bbase = t ? atomicAdd(count, t) : 0u;
+    }
+    __syncthreads();
+    unsigned int base = bbase;
+    for (int k = 0; k < w; ++k) base += wcount[k];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < kSelectPer; ++r) {
+        if ((masks[r] >> lane_id()) & 1ull)
+            emit(i0 + r * kWave, base + static_cast<unsigned int>(__popcll(masks[r] & lt)));
+        base += static_cast<unsigned int>(__popcll(masks[r]));
+    }
+}
+
+// mode 0: boxes with cores (v = bmincore != INT_MAX); mode 1: roots (v[b] == b)
 __global__ __launch_bounds__(kThreads) void k_select_boxes(int32_t nbox, int mode,
                                                            const int32_t* __restrict__ v,
                                                            int32_t* __restrict__ list,
                                                            unsigned int* __restrict__ count) {
-    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    bool take = false;
-    if (b < nbox) take = mode == 0 ? v[b] != INT_MAX : v[b] == b;
-    const uint64_t mask = ballot(take);
-    if (!mask) return;
-    unsigned int base = 0;
-    if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned int>(__popcll(mask)));
-    base = __shfl(base, 0, 64);
-    if (take) list[base + __popcll(mask & lanemask_lt())] = b;
+    block_select(
+        nbox, count,
+        [&](int64_t b) { return mode == 0 ? v[b] != INT_MAX : v[b] == static_cast<int32_t>(b); },
+        [&](int64_t b, unsigned int pos) { list[pos] = static_cast<int32_t>(b); });
 }
 
 // the 64-slot chunks of the boxes that have open pairs (off[b+1] > off[b])
@@ -512,19 +552,13 @@ __global__ __launch_bounds__(kThreads) void k_select_items(int32_t nitems,
                                                            const uint32_t* __restrict__ off,
                                                            int2* __restrict__ out,
                                                            unsigned int* __restrict__ count) {
-    const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    bool take = false;
-    int2 it = make_int2(0, 0);
-    if (i < nitems) {
-        it = items[i];
-        take = off[it.x + 1] > off[it.x];
-    }
-    const uint64_t mask = ballot(take);
-    if (!mask) return;
-    unsigned int base = 0;
-    if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned int>(__popcll(mask)));
-    base = __shfl(base, 0, 64);
-    if (take) out[base + __popcll(mask & lanemask_lt())] = it;
+    block_select(
+        nitems, count,
+        [&](int64_t i) {
+            const int32_t b = items[i].x;
+            return off[b + 1] > off[b];
+        },
+        [&](int64_t i, unsigned int pos) { out[pos] = items[i]; });
 }
 
 // root of every box's hook tree -> rsnap (-1 for boxes without cores)
@@ -820,7 +854,7 @@ void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2*
 void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list,
                          unsigned int* count, hipStream_t s) {
     if (nbox <= 0) return;
-    hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox, kSelectSpan)), dim3(kThreads), 0, s,
                        nbox, mode, v, list, count);
     HPD_HIP(hipGetLastError());
 }
@@ -828,8 +862,8 @@ void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list
 void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
                          unsigned int* count, hipStream_t s) {
     if (nitems <= 0) return;
-    hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems, kThreads)), dim3(kThreads), 0, s,
-                       nitems, items, off, out, count);
+    hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems, kSelectSpan)), dim3(kThreads), 0,
+                       s, nitems, items, off, out, count);
     HPD_HIP(hipGetLastError());
 }
 

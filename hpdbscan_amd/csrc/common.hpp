@@ -130,6 +130,25 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : ((~0ull) >> (64 - l));
 }
 
+// Adds v of every thread of the block to *out with one atomic per block
+// (wave sums meet in LDS).  Every thread of the block must call it.  One
+// atomic per wave on one address serialises a kernel of thousands of waves.
+template <int kBlockThreads>
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* out) {
+    constexpr int kWaves = kBlockThreads / 64;
+    __shared__ unsigned long long part[kWaves];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += part[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Lock-free union-find over sorted positions.  parent[x] <= x always; a root
 // is hooked only under a smaller root, so every tree's root is its smallest

@@ -174,10 +174,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                      : 0u;
     }
     // Σ over rows of the cell of |candidates| = count_u * cand
-    unsigned long long v = static_cast<unsigned long long>(cand * count_u);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (__lane_id() == 0 && v) atomicAdd(sumcand, v);
+    block_add<256>(static_cast<unsigned long long>(cand * count_u), sumcand);
 }
 
 template <typename K, typename T>
