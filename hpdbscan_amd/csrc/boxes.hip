@@ -315,8 +315,9 @@ __global__ __launch_bounds__(kThreads) void k_box_scan(
     const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ rsnap,
     uint32_t* __restrict__ cnt, int32_t* __restrict__ list, unsigned long long* __restrict__ total) {
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    const int32_t sb = b < nbox ? rsnap[b] : -1;
     uint32_t k = (list && b < nbox) ? cnt[b] : 0u;
+    // the fill pass walks only the boxes the count pass found pairs for
+    const int32_t sb = b < nbox && (!list || cnt[b + 1] > k) ? rsnap[b] : -1;
     if (sb >= 0) {
         const int32_t u = bcell[b];
         const double gap2_max = jobs[job_of_cell[u]].gap2_max;

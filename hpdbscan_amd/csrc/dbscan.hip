@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
 // few ulps of d^2, so gmin2 > EPS2 (1 + 1e-5) (+ 1e-30) rules every member
 // out and gmax2 <= EPS2 (1 - 1e-5) rules every member in.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_core_boxes(const DbscanArgs<T> a,
+__global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> a,
                                                          uint8_t* __restrict__ core,
                                                          unsigned long long* __restrict__ ncore) {
     Chunk c;
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
 // clusters; the smallest adjacent cluster and the two-cluster flag come out
 // exact.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_border_boxes(const DbscanArgs<T> a,
+__global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T> a,
                                                            const uint8_t* __restrict__ core,
                                                            const int32_t* __restrict__ bcluster,
                                                            const uint32_t* __restrict__ order,
