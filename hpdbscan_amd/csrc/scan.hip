@@ -1,4 +1,6 @@
-// Exclusive prefix sum over uint32 (reduce / scan-of-sums / downsweep).
+// Exclusive prefix sum over uint32: a reduce of 4096-element tiles, then a
+// downsweep whose blocks add their tile offsets (two launches up to 16.7M
+// elements; beyond, the tile sums are scanned recursively in between).
 // Used for the radix-sort digit offsets, the occupied-cell numbering and the
 // CSR offsets of the neighbour ranges.
 #include "common.hpp"
@@ -58,12 +60,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
 }
 
 // Scans one tile per block: thread t owns elements [t*16, t*16+16) of the
-// tile, adds the tile offset (offs[blockIdx.x], or 0 when offs is null).
+// tile.  The tile's offset is offs[blockIdx.x] (offs = scanned tile sums),
+// or, with sums instead, the block's own sum of sums[0, blockIdx.x) (the
+// raw tile sums; used up to kScanTile tiles, where that is at most 16 loads
+// per thread and saves the separate scan of the sums), or 0 when both are
+// null.  The thread holding element n - 1 writes the total to total_dev
+// (input and output may alias: every block reads its tile before writing).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, uint32_t* out,
                                                             int64_t n,
-                                                            const uint32_t* __restrict__ offs) {
+                                                            const uint32_t* __restrict__ offs,
+                                                            const uint32_t* __restrict__ sums,
+                                                            uint32_t* __restrict__ total_dev) {
     __shared__ uint32_t tile[kScanTile];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    uint32_t off = offs ? offs[blockIdx.x] : 0u;
+    if (sums) {
+        uint32_t acc = 0;
+        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) acc += sums[j];
+        (void)block_exclusive_scan(acc, &off);
+    }
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const int idx = i * kScanThreads + threadIdx.x;
@@ -79,10 +94,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, 
         acc += v[i];
     }
     uint32_t tot;
-    uint32_t run = block_exclusive_scan(acc, &tot) + (offs ? offs[blockIdx.x] : 0u);
+    uint32_t run = block_exclusive_scan(acc, &tot) + off;
+    const int64_t last = n - 1 - base;  // tile index of element n - 1
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        tile[threadIdx.x * kScanItems + i] = run;
+        const int idx = static_cast<int>(threadIdx.x) * kScanItems + i;
+        tile[idx] = run;
+        if (total_dev && idx == last) *total_dev = run + v[i];
         run += v[i];
     }
     __syncthreads();
@@ -94,15 +112,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, 
     }
 }
 
-// Totals need the last input value, which an in-place scan overwrites; this
-// kernel runs before the downsweep and stashes it.
-__global__ void k_scan_stash_last(const uint32_t* __restrict__ in, int64_t n, uint32_t* stash) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *stash = n > 0 ? in[n - 1] : 0u;
-}
-
-__global__ void k_scan_finish_total(const uint32_t* __restrict__ out, int64_t n,
-                                    const uint32_t* stash, uint32_t* total_dev) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total_dev = (n > 0 ? out[n - 1] : 0u) + *stash;
+__global__ void k_scan_zero_total(uint32_t* total_dev) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total_dev = 0u;
 }
 
 int64_t ntiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
@@ -110,7 +121,7 @@ int64_t ntiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 }  // namespace
 
 size_t scan_tmp_words(int64_t n) {
-    size_t words = 4;  // stash + padding
+    size_t words = 4;  // padding
     int64_t m = n;
     while (m > kScanTile) {
         m = ntiles(m);
@@ -119,11 +130,16 @@ size_t scan_tmp_words(int64_t n) {
     return words;
 }
 
-static void scan_rec(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, hipStream_t s) {
-    if (n <= 0) return;
+static void scan_rec(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                     uint32_t* total_dev, hipStream_t s) {
+    const uint32_t* none = nullptr;
+    if (n <= 0) {
+        if (total_dev) hipLaunchKernelGGL(k_scan_zero_total, dim3(1), dim3(64), 0, s, total_dev);
+        return;
+    }
     if (n <= kScanTile) {
-        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanThreads), 0, s, in, out, n,
-                           static_cast<const uint32_t*>(nullptr));
+        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanThreads), 0, s, in, out, n, none, none,
+                           total_dev);
         return;
     }
     const int64_t nb = ntiles(n);
@@ -131,19 +147,20 @@ static void scan_rec(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp
     uint32_t* rest = tmp + align_up(static_cast<size_t>(nb), 64);
     hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in,
                        n, sums);
-    scan_rec(sums, sums, nb, rest, s);
+    if (nb <= kScanTile) {
+        // each block sums its predecessors' tile sums itself
+        hipLaunchKernelGGL(k_scan_down, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s,
+                           in, out, n, none, static_cast<const uint32_t*>(sums), total_dev);
+        return;
+    }
+    scan_rec(sums, sums, nb, rest, nullptr, s);
     hipLaunchKernelGGL(k_scan_down, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in,
-                       out, n, static_cast<const uint32_t*>(sums));
+                       out, n, static_cast<const uint32_t*>(sums), none, total_dev);
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                         uint32_t* total_dev, hipStream_t s) {
-    uint32_t* stash = tmp;
-    if (total_dev) hipLaunchKernelGGL(k_scan_stash_last, dim3(1), dim3(64), 0, s, in, n, stash);
-    scan_rec(in, out, n, tmp + 4, s);
-    if (total_dev)
-        hipLaunchKernelGGL(k_scan_finish_total, dim3(1), dim3(64), 0, s, out, n,
-                           static_cast<const uint32_t*>(stash), total_dev);
+    scan_rec(in, out, n, tmp + 4, total_dev, s);
     HPD_HIP(hipGetLastError());
 }
 
