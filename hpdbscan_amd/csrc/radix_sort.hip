@@ -6,7 +6,9 @@
 //                      hist[digit * ntiles + tile]
 //   2. exclusive scan  of hist -> global offset of (digit, tile)
 //   3. k_radix_scatter stable rank of every key inside its tile (wave
-//                      ballots + per-wave LDS counters), then the write
+//                      ballots + per-wave LDS counters), the tile staged
+//                      in LDS in digit order, then written out so that
+//                      consecutive threads write consecutive places
 // A tile is 4096 keys: 4 waves x 16 iterations x 64 lanes; the stable order
 // inside a tile is (wave, iteration, lane), i.e. input order.
 #include "common.hpp"
@@ -55,6 +57,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
                                                             int64_t ntiles) {
     __shared__ uint32_t cnt[4][kBuckets];
     __shared__ uint32_t gbase[kBuckets];
+    __shared__ uint32_t tbase[kBuckets];  // digit's first place in the staged tile
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ K skey[kTile];
+    __shared__ uint32_t sval[kTile];
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
 #pragma unroll
@@ -90,23 +96,53 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
         val[it] = valid ? vin[idx] : 0u;
     }
     __syncthreads();
-    {
-        // exclusive prefix of each digit over the four waves
-        uint32_t run = 0;
+    // exclusive prefix of each digit over the four waves, and of the digits'
+    // tile totals over the digits (thread t owns digit t)
+    uint32_t run = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t c = cnt[i][threadIdx.x];
-            cnt[i][threadIdx.x] = run;
-            run += c;
-        }
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t c = cnt[i][threadIdx.x];
+        cnt[i][threadIdx.x] = run;
+        run += c;
     }
+    uint32_t inc = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (l >= o) inc += t;
+    }
+    if (l == 63) wsum[w] = inc;
     __syncthreads();
+    uint32_t woff = 0;
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i)
+        if (i < w) woff += wsum[i];
+    tbase[threadIdx.x] = woff + inc - run;
+    __syncthreads();
+    // stage the tile in digit order (stable: waves, iterations and lanes are
+    // in input order) ...
 #pragma unroll
     for (int it = 0; it < kIters; ++it) {
         if (dgt[it] != 0xffffffffu) {
-            const uint32_t dst = gbase[dgt[it]] + cnt[w][dgt[it]] + rank[it];
-            kout[dst] = key[it];
-            vout[dst] = val[it];
+            const uint32_t at = tbase[dgt[it]] + cnt[w][dgt[it]] + rank[it];
+            skey[at] = key[it];
+            sval[at] = val[it];
+        }
+    }
+    __syncthreads();
+    // ... and write it out: consecutive threads write consecutive places of
+    // one digit's run
+    const int64_t left = n - static_cast<int64_t>(blockIdx.x) * kTile;
+    const int valid_in_tile = left < kTile ? static_cast<int>(left) : kTile;
+#pragma unroll
+    for (int it = 0; it < kIters; ++it) {
+        const int at = it * kThreads + threadIdx.x;
+        if (at < valid_in_tile) {
+            const K k = skey[at];
+            const uint32_t d = static_cast<uint32_t>((k >> shift) & 0xff);
+            const uint32_t dst = gbase[d] + (static_cast<uint32_t>(at) - tbase[d]);
+            kout[dst] = k;
+            vout[dst] = sval[at];
         }
     }
 }
