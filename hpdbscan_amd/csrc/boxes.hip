@@ -666,11 +666,16 @@ __global__ __launch_bounds__(kThreads) void k_box_compmin(int32_t nbox,
                                                           int32_t* bparent,
                                                           unsigned long long* __restrict__ cmin) {
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (b >= nbox || bmincore[b] == INT_MAX) return;
-    const int32_t r = uf_find(bparent, b);
-    const unsigned long long key = (static_cast<unsigned long long>(bcell[b]) << 32) |
-                                   static_cast<uint32_t>(bmincore[b]);
-    atomicMin(&cmin[r], key);
+    const bool act = b < nbox && bmincore[b] != INT_MAX;
+    // neighbouring boxes mostly share their root: lanes of a run of equal
+    // roots combine first, and only the run's first lane takes the atomic
+    const int32_t r = act ? uf_find(bparent, b) : -1 - lane_id();
+    unsigned long long key = act ? (static_cast<unsigned long long>(bcell[b]) << 32) |
+                                       static_cast<uint32_t>(bmincore[b])
+                                 : ~0ull;
+    key = run_suffix_min<unsigned long long>(key, r);
+    const bool head = run_head(r);  // every lane takes part in the shuffles
+    if (act && head) atomicMin(&cmin[r], key);
 }
 
 // number of entries < row in the ascending v[lo, hi)

@@ -560,7 +560,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     const bool one_thread = ctx.border_rule == HPD_BORDER_ONE_THREAD;
     int2* amb = one_thread ? A.take_n<int2>(nn) : nullptr;
     launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
-                          cmax, labels, s);
+                          cmax, labels, &dstats[1], s);
     tm.mark(M_BORDER0);
     launch_border<T, D>(a, core, box_union ? bcluster : nullptr, rootslot, cmin, cmax, labels,
                         &dstats[2], amb, s);

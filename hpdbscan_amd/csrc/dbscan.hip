@@ -32,6 +32,7 @@
 // Work decomposition: one wave per 64-row chunk of an occupied cell; lanes
 // own the chunk's rows, the candidate rows stream through the wave in
 // aligned batches fetched with wave-uniform addresses.
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 
@@ -366,20 +367,30 @@ __global__ __launch_bounds__(kThreads) void k_label_cores(
 
 // Box mode: core labels one thread per slot (the cluster of a core is its
 // box's); cmin / cmax are not needed there.
+// 8 waves on each of the 1024 SIMDs
+constexpr unsigned kLabelBlocks = 2048;
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
     int64_t n, const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ job_of_cell,
     const uint8_t* __restrict__ core, const int32_t* __restrict__ box_of_slot,
     const int32_t* __restrict__ bcell, const int32_t* __restrict__ bcluster,
     const uint32_t* __restrict__ inrow, int64_t* __restrict__ labels,
-    int32_t* __restrict__ rootslot) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (p >= n || !core[p]) return;
-    const int32_t b = box_of_slot[p];
-    const int32_t r = bcluster[b];
-    rootslot[p] = r;
-    const int64_t first_pos = jobs[job_of_cell[bcell[b]]].row_begin;
-    labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
+    int32_t* __restrict__ rootslot, unsigned long long* __restrict__ ncore) {
+    // grid-stride over at most kLabelBlocks blocks, so that the core count
+    // of the run statistics (folded in here) takes few atomics
+    unsigned long long cores = 0;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; p < n;
+         p += static_cast<int64_t>(gridDim.x) * kThreads) {
+        if (!core[p]) continue;
+        ++cores;
+        const int32_t b = box_of_slot[p];
+        const int32_t r = bcluster[b];
+        rootslot[p] = r;
+        const int64_t first_pos = jobs[job_of_cell[bcell[b]]].row_begin;
+        labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
+    }
+    if (ncore) block_add<kThreads>(cores, ncore);
 }
 
 template <typename T, int D>
@@ -843,14 +854,12 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
             if (ac.nsitems > 0)
                 hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(ac.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, ac, core, ncore);
-            count_flags(core, a.n, ncore, s);
             return;
         }
     }
     hipLaunchKernelGGL((k_core<T, D>), dim3(blocks_for(a.nitems, kWavesPerBlock)), dim3(kThreads),
                        0, s, a, core, ncore);
     HPD_HIP(hipGetLastError());
-    count_flags(core, a.n, ncore, s);
 }
 
 template <typename T, int D>
@@ -878,16 +887,18 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                        int32_t* cmax, int64_t* labels, hipStream_t s) {
+                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s) {
     if (a.n <= 0) return;
     if (bcluster && a.bcell) {
-        hipLaunchKernelGGL(k_label_cores_boxes<T>, dim3(blocks_for(a.n, kThreads)), dim3(kThreads),
-                           0, s, a.n, a.jobs, a.job_of_cell, core, a.box_of_slot, a.bcell, bcluster,
-                           a.inrow, labels, rootslot);
+        const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
+        hipLaunchKernelGGL(k_label_cores_boxes<T>, dim3(grid), dim3(kThreads), 0, s, a.n, a.jobs,
+                           a.job_of_cell, core, a.box_of_slot, a.bcell, bcluster, a.inrow, labels,
+                           rootslot, ncore);
         HPD_HIP(hipGetLastError());
         return;
     }
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
+    count_flags(core, a.n, ncore, s);
     hipLaunchKernelGGL(k_fill2, dim3(blocks_for(a.m, kThreads)), dim3(kThreads), 0, s, cmin, INT_MAX,
                        cmax, -1, static_cast<int64_t>(a.m));
     hipLaunchKernelGGL(k_label_cores<T>, dim3(gw), dim3(kThreads), 0, s, a.items, a.nitems,
@@ -984,7 +995,7 @@ HPD_FOR_EACH_DTYPE(HPD_INST_DBSCAN_ALL_D)
 #define HPD_INST_LABEL_CORES(T)                                                                \
     template void launch_label_cores<T>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,  \
                                         int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, \
-                                        int64_t*, hipStream_t);
+                                        int64_t*, unsigned long long*, hipStream_t);
 HPD_FOR_EACH_DTYPE(HPD_INST_LABEL_CORES)
 
 }  // namespace hpd

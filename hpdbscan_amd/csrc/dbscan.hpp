@@ -54,7 +54,8 @@ struct DbscanArgs {
 };
 
 // core flag of every slot; with box tables the candidates are visited box
-// by box (boxes beyond eps skipped, boxes within eps counted whole)
+// by box (boxes beyond eps skipped, boxes within eps counted whole).  The
+// number of cores is added to *ncore by launch_label_cores.
 template <typename T, int D>
 void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s);
 
@@ -71,12 +72,12 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 // bcluster (box mode, bcluster != null) or compmin[find(parent, slot)];
 // rootslot[slot] = cluster, cmin / cmax = smallest / largest cluster per
 // occupied cell (not in box mode, where the border pass reads bcluster).
-// Labels go to input order through a.inrow.
+// Labels go to input order through a.inrow; *ncore += number of cores.
 // rootslot: ld entries.
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                        int32_t* cmax, int64_t* labels, hipStream_t s);
+                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s);
 
 // Border and noise labels of the non-core rows (in box mode box by box:
 // all cores of a box share its cluster).  stats[0] += noise rows,
