@@ -224,6 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
+    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
     T pc[D];
     load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
     const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
@@ -239,12 +240,15 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
             point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
             const bool act = pcore && gmin2 <= gap2_max;
             if (!__any(act)) return false;
-            bool hit = false;
-            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
-            };
-            auto stop = [&] { return static_cast<bool>(__any(hit)); };
-            if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+            // a core row within eps of the whole box is within eps of its cores
+            bool hit = act && gmax2 <= full2;
+            if (!__any(hit)) {
+                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                    if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+                };
+                auto stop = [&] { return static_cast<bool>(__any(hit)); };
+                if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+            }
             if (lane_id() == 0) bparent[b] = b2;
             return true;
         },
@@ -452,6 +456,13 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
     const JobParams<T>& jp = jobs[job_of_cell[u]];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
+    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+    double blo[D], bhi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        blo[j] = box_lo<T, D>(bbox, b, j);
+        bhi[j] = box_hi<T, D>(bbox, b, j);
+    }
     const int32_t r0 = static_cast<int32_t>(roff[u]), r1 = static_cast<int32_t>(roff[u + 1]);
     const int32_t center = rcenter[u];
     for (int32_t k = -1; k < r1 - r0; ++k) {
@@ -463,6 +474,22 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
         const int32_t e1 = min(cbox[cs.y], b);
         for (int32_t b2 = e0; b2 < e1; ++b2) {
             if (bmincore[b2] == INT_MAX) continue;
+            // the two bounding boxes: beyond eps of each other, nothing to
+            // test; every pair within eps, the boxes' cores meet
+            double gap2 = 0.0, far2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const double lo = box_lo<T, D>(bbox, b2, j), hi = box_hi<T, D>(bbox, b2, j);
+                const double g = fmax(0.0, fmax(lo - bhi[j], blo[j] - hi));
+                const double f = fmax(hi - blo[j], bhi[j] - lo);
+                gap2 += g * g;
+                far2 += f * f;
+            }
+            if (gap2 > gap2_max) continue;
+            if (far2 <= full2) {
+                bparent[b] = b2;
+                return;
+            }
             const int32_t t0 = bstart[b2], t1 = bstart[b2 + 1];
             int tests = 0;
             for (int32_t p = s0; p < s1 && tests < kSmallTests; ++p) {
@@ -472,6 +499,10 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
                 double gmin2, gmax2;
                 point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
                 if (gmin2 > gap2_max) continue;
+                if (gmax2 <= full2) {  // within eps of all of b2, its cores included
+                    bparent[b] = b2;
+                    return;
+                }
                 for (int32_t q = t0; q < t1 && tests < kSmallTests; ++q) {
                     if (!core[q]) continue;
                     ++tests;
@@ -487,8 +518,6 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
     }
 }
 
-// list[...] = every box b with (mode 0) cores / (mode 1) rsnap[b] == b (a
-// root), in no particular order; *count += their number
 // Stream compaction with one atomic per block: block `blockIdx.x` takes
 // the kSelectPer * kThreads consecutive indices from blockIdx.x *
 // kSelectSpan, wave w the 64 * kSelectPer of them from w * 64 * kSelectPer,
@@ -536,7 +565,8 @@ __device__ __forceinline__ void block_select(int64_t n, unsigned int* count, Tak
     }
 }
 
-// mode 0: boxes with cores (v = bmincore != INT_MAX); mode 1: roots (v[b] == b)
+// list[...] = every box b with (mode 0) cores (v = bmincore != INT_MAX) /
+// (mode 1) v[b] == b (a root); *count += their number
 __global__ __launch_bounds__(kThreads) void k_select_boxes(int32_t nbox, int mode,
                                                            const int32_t* __restrict__ v,
                                                            int32_t* __restrict__ list,
