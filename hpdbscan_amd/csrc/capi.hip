@@ -379,8 +379,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
 
     // neighbour ranges (spatial_index.h:682-737) and work items
     uint32_t* ccnt = boxes ? A.take_n<uint32_t>(nn + 1) : nullptr;
+    // the count pass keeps the rows it finds in scratch64 (idle until the
+    // union) when every cell's 3^(D-1) rows fit, and the fill copies them
+    int32_t stash_rows = 1;
+    for (int j = 1; j < D; ++j) stash_rows *= 3;
+    int2* stash = scratch64 && static_cast<uint64_t>(m) * static_cast<uint64_t>(stash_rows) <= nn
+                      ? reinterpret_cast<int2*>(scratch64)
+                      : nullptr;
     launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
-                                &dstats[0], s);
+                                &dstats[0], stash, stash_rows, rcenter, s);
     exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
     exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
@@ -430,8 +437,11 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
     int2* uitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
     int2* citems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
-    launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
-                               rcenter, s);
+    if (stash)
+        launch_neighbor_copy(stash, stash_rows, ustart, m, nranges, ranges, rcells, rcenter, s);
+    else
+        launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
+                                   rcenter, s);
     launch_work_items(ustart, m, nchunks, nitems, items, s);
     if (boxes) {
         launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, nditems, sitems, s);

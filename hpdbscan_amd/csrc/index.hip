@@ -146,7 +146,10 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              int32_t* __restrict__ job_of_cell,
                                                              uint32_t* __restrict__ nranges,
                                                              uint32_t* __restrict__ core_chunks,
-                                                             unsigned long long* __restrict__ sumcand) {
+                                                             unsigned long long* __restrict__ sumcand,
+                                                             int2* __restrict__ stash,
+                                                             int32_t stash_rows,
+                                                             int32_t* __restrict__ center_local) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
@@ -156,16 +159,22 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         job_of_cell[u] = j;
         const uint64_t base = jobs[j].key_base;
         uint32_t cnt = 0;
-        int32_t from = 0;
-        for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool) {
+        int32_t from = 0, center = 0;
+        int2* mine = stash ? stash + static_cast<int64_t>(u) * stash_rows : nullptr;
+        for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool is_center) {
             const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
             from = ab.y;
             if (ab.y > ab.x) {
+                if (mine) {
+                    if (is_center) center = static_cast<int32_t>(cnt);
+                    mine[cnt] = ab;
+                }
                 ++cnt;
                 cand += static_cast<uint64_t>(ustart[ab.y] - ustart[ab.x]);
             }
         });
         nranges[u] = cnt;
+        if (mine) center_local[u] = center;
         count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
         // the reference queries a row only with >= minPts candidates
         // (hpdbscan.h:74); with fewer, no row of the cell can be core
@@ -205,6 +214,28 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
                      }
                  });
     rcenter[u] = center;
+}
+
+// The fill from the count pass's stash: the same ranges, rcells and rcenter
+// as k_neighbor_rows_fill, without searching again.
+__global__ __launch_bounds__(256) void k_neighbor_rows_copy(const int2* __restrict__ stash,
+                                                            int32_t stash_rows,
+                                                            const int32_t* __restrict__ ustart,
+                                                            int32_t m,
+                                                            const uint32_t* __restrict__ roff,
+                                                            int2* __restrict__ ranges,
+                                                            int2* __restrict__ rcells,
+                                                            int32_t* rcenter) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const uint32_t w0 = roff[u], cnt = roff[u + 1] - w0;
+    const int2* mine = stash + static_cast<int64_t>(u) * stash_rows;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const int2 ab = mine[k];
+        rcells[w0 + k] = ab;
+        ranges[w0 + k] = make_int2(ustart[ab.x], ustart[ab.y]);
+    }
+    rcenter[u] = static_cast<int32_t>(w0) + rcenter[u];  // local index from the count pass
 }
 
 __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
@@ -307,9 +338,18 @@ template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           hipStream_t s) {
+                           int2* stash, int32_t stash_rows, int32_t* rcenter, hipStream_t s) {
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
-                       ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand);
+                       ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
+                       stash, stash_rows, rcenter);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
+                          const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbor_rows_copy, dim3(blocks_for(m, 256)), dim3(256), 0, s, stash,
+                       stash_rows, ustart, m, roff, ranges, rcells, rcenter);
     HPD_HIP(hipGetLastError());
 }
 
@@ -344,11 +384,11 @@ HPD_INST_INDEX(uint64_t)
     template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     hipStream_t);                              \
+                                                     int2*, int32_t, int32_t*, hipStream_t);    \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     hipStream_t);                              \
+                                                     int2*, int32_t, int32_t*, hipStream_t);    \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \

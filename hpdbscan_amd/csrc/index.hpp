@@ -19,12 +19,20 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 // job_of_cell[u] = job of cell u; nranges[u] = number of non-empty
 // neighbour rows of cell u; core_chunks[u] (when not null) = 64-row chunks
 // of cell u if it has >= minPts candidates, else 0; adds
-// Σ_u count(u) * |candidates(u)| to *sumcand.
+// Σ_u count(u) * |candidates(u)| to *sumcand.  With a stash (m * stash_rows
+// entries, stash_rows >= the most neighbour rows a cell can have) the
+// rows' cell ranges go to stash[u * stash_rows ..] and the own row's index
+// among them to rcenter[u], for launch_neighbor_copy.
 template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           hipStream_t s);
+                           int2* stash, int32_t stash_rows, int32_t* rcenter, hipStream_t s);
+
+// launch_neighbor_fill's output from the count pass's stash (roff scanned)
+void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
+                          const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
+                          hipStream_t s);
 
 // ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row,
 // ascending; rcells[..] = [first, end) occupied-cell index of the same row;
