@@ -301,7 +301,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     uint32_t* vals_b = A.take_n<uint32_t>(nn);
     const size_t hist_words = radix_hist_words(n);
     uint32_t* hist = A.take_n<uint32_t>(hist_words);
-    uint32_t* tmp = A.take_n<uint32_t>(scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2))));
+    const size_t tmp_words = scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2)));
+    uint32_t* tmp = A.take_n<uint32_t>(tmp_words);
     T* soa = A.take_n<T>(L * D);
     uint32_t* flags = A.take_n<uint32_t>(2 * nn + 2);
     K* ucell = A.take_n<K>(nn);
@@ -388,19 +389,25 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                       : nullptr;
     launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
                                 &dstats[0], stash, stash_rows, rcenter, s);
-    exclusive_scan_u32(nranges, nranges, m, tmp, &nranges[m], s);
     launch_chunk_counts(ustart, m, nchunks, s);
-    exclusive_scan_u32(nchunks, nchunks, m, tmp, &nchunks[m], s);
     // box mode: dense-cell chunks and sparse-cell rows as separate work
     // lists; their per-cell counts live in the (now unused) run flags
     uint32_t* dcnt = flags;
     uint32_t* scnt = flags + nn + 1;
+    if (boxes) launch_split_counts(ustart, m, ctx.sparse_max, dcnt, scnt, s);
+    // the per-cell counts, scanned together (totals at entry m)
+    ScanSet counts{};
+    int ncounts = 0;
+    counts.a[ncounts++] = nranges;
+    counts.a[ncounts++] = nchunks;
     if (boxes) {
-        exclusive_scan_u32(ccnt, ccnt, m, tmp, &ccnt[m], s);
+        counts.a[ncounts++] = ccnt;
+        counts.a[ncounts++] = dcnt;
+        counts.a[ncounts++] = scnt;
+    }
+    exclusive_scan_set_u32(counts, ncounts, m, tmp, tmp_words, s);
+    if (boxes) {
         HPD_HIP(hipMemcpyAsync(&counters[11], &ccnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        launch_split_counts(ustart, m, ctx.sparse_max, dcnt, scnt, s);
-        exclusive_scan_u32(dcnt, dcnt, m, tmp, &dcnt[m], s);
-        exclusive_scan_u32(scnt, scnt, m, tmp, &scnt[m], s);
         HPD_HIP(hipMemcpyAsync(&counters[6], &dcnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         HPD_HIP(hipMemcpyAsync(&counters[7], &scnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     }

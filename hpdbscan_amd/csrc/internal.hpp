@@ -58,6 +58,17 @@ size_t scan_tmp_words(int64_t n);
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                         uint32_t* total_dev, hipStream_t s);
 
+// In-place exclusive scans of k <= kMaxScanSet arrays of one length n, each
+// array's total to its entry n (every array holds n + 1 entries), in two
+// launches for all of them; tmp holds tmp_words uint32 (separate scans
+// where that is too few or n exceeds 4096 * 4096).
+constexpr int kMaxScanSet = 8;
+struct ScanSet {
+    uint32_t* a[kMaxScanSet];
+};
+void exclusive_scan_set_u32(const ScanSet& set, int k, int64_t n, uint32_t* tmp, size_t tmp_words,
+                            hipStream_t s);
+
 // Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the
 // key.  Returns true when the result is in (keys_b, vals_b), false when it is
 // in (keys_a, vals_a).  `hist` must hold radix_hist_words(n) uint32 and `tmp`

@@ -66,17 +66,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
 // per thread and saves the separate scan of the sums), or 0 when both are
 // null.  The thread holding element n - 1 writes the total to total_dev
 // (input and output may alias: every block reads its tile before writing).
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, uint32_t* out,
-                                                            int64_t n,
-                                                            const uint32_t* __restrict__ offs,
-                                                            const uint32_t* __restrict__ sums,
-                                                            uint32_t* __restrict__ total_dev) {
+__device__ __forceinline__ void scan_tile(const uint32_t* in, uint32_t* out, int64_t n,
+                                          const uint32_t* __restrict__ offs,
+                                          const uint32_t* __restrict__ sums,
+                                          uint32_t* __restrict__ total_dev, uint32_t blk) {
     __shared__ uint32_t tile[kScanTile];
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
-    uint32_t off = offs ? offs[blockIdx.x] : 0u;
+    const int64_t base = static_cast<int64_t>(blk) * kScanTile;
+    uint32_t off = offs ? offs[blk] : 0u;
     if (sums) {
         uint32_t acc = 0;
-        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) acc += sums[j];
+        for (uint32_t j = threadIdx.x; j < blk; j += kScanThreads) acc += sums[j];
         (void)block_exclusive_scan(acc, &off);
     }
 #pragma unroll
@@ -110,6 +109,46 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, 
         const int64_t g = base + idx;
         if (g < n) out[g] = tile[idx];
     }
+}
+
+// Scans one tile per block: thread t owns elements [t*16, t*16+16) of the
+// tile.  The tile's offset is offs[blockIdx.x] (offs = scanned tile sums),
+// or, with sums instead, the block's own sum of sums[0, blockIdx.x) (the
+// raw tile sums; used up to kScanTile tiles, where that is at most 16 loads
+// per thread and saves the separate scan of the sums), or 0 when both are
+// null.  The thread holding element n - 1 writes the total to total_dev
+// (input and output may alias: every block reads its tile before writing).
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, uint32_t* out,
+                                                            int64_t n,
+                                                            const uint32_t* __restrict__ offs,
+                                                            const uint32_t* __restrict__ sums,
+                                                            uint32_t* __restrict__ total_dev) {
+    scan_tile(in, out, n, offs, sums, total_dev, blockIdx.x);
+}
+
+// In-place scans of several arrays of one length n (array blockIdx.y), the
+// total of each to its entry n.
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce_set(ScanSet set, int64_t n,
+                                                                  uint32_t* __restrict__ sums,
+                                                                  int64_t stride) {
+    const uint32_t* in = set.a[blockIdx.y];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t idx = base + static_cast<int64_t>(i) * kScanThreads + threadIdx.x;
+        if (idx < n) acc += in[idx];
+    }
+    uint32_t tot;
+    (void)block_exclusive_scan(acc, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.y * stride + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_set(ScanSet set, int64_t n,
+                                                                const uint32_t* __restrict__ sums,
+                                                                int64_t stride) {
+    uint32_t* a = set.a[blockIdx.y];
+    scan_tile(a, a, n, nullptr, sums ? sums + blockIdx.y * stride : nullptr, a + n, blockIdx.x);
 }
 
 __global__ void k_scan_zero_total(uint32_t* total_dev) {
@@ -161,6 +200,24 @@ static void scan_rec(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                         uint32_t* total_dev, hipStream_t s) {
     scan_rec(in, out, n, tmp + 4, total_dev, s);
+    HPD_HIP(hipGetLastError());
+}
+
+void exclusive_scan_set_u32(const ScanSet& set, int k, int64_t n, uint32_t* tmp, size_t tmp_words,
+                            hipStream_t s) {
+    const int64_t nb = ntiles(n);
+    if (n <= 0 || k <= 0 || k > kMaxScanSet || nb > kScanTile ||
+        static_cast<size_t>(k) * align_up(static_cast<size_t>(nb), 64) + 4 > tmp_words) {
+        for (int i = 0; i < k; ++i) exclusive_scan_u32(set.a[i], set.a[i], n, tmp, set.a[i] + n, s);
+        return;
+    }
+    const int64_t stride = static_cast<int64_t>(align_up(static_cast<size_t>(nb), 64));
+    uint32_t* sums = tmp + 4;
+    if (nb > 1)
+        hipLaunchKernelGGL(k_scan_reduce_set, dim3(static_cast<unsigned>(nb), k), dim3(kScanThreads),
+                           0, s, set, n, sums, stride);
+    hipLaunchKernelGGL(k_scan_down_set, dim3(static_cast<unsigned>(nb), k), dim3(kScanThreads), 0, s,
+                       set, n, static_cast<const uint32_t*>(nb > 1 ? sums : nullptr), stride);
     HPD_HIP(hipGetLastError());
 }
 
