@@ -283,6 +283,7 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     b += sizeof(T) * 2 * D * (mm_blocks + nj) + 4 * (mm_blocks + nj + 2) + 64;  // min/max
     b += sizeof(JobParams<T>) * nj + 4 * nn;   // job table, job of every cell
     if (one_thread) b += 8 * nn + 4 * nn + nn + 4;  // ambiguous rows, f, involved, flag
+    b += 8 * (16 + kNoiseStripes);            // statistics, counters, noise counters
     b += 52 * 256 + 64 * 1024;                 // alignment
     return b;
 }
@@ -317,9 +318,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int32_t* cmax = A.take_n<int32_t>(nn);
     int32_t* rcenter = A.take_n<int32_t>(nn);
     int32_t* job_of_cell = A.take_n<int32_t>(nn);
-    unsigned long long* dstats = A.take_n<unsigned long long>(16);
+    unsigned long long* dstats = A.take_n<unsigned long long>(16 + kNoiseStripes);
     uint32_t* counters = reinterpret_cast<uint32_t*>(dstats + 8);  // m, ranges, items, boxes, box items, loose
-    HPD_HIP(hipMemsetAsync(dstats, 0, 16 * sizeof(unsigned long long), s));
+    unsigned long long* noise_stripes = dstats + 16;
+    HPD_HIP(hipMemsetAsync(dstats, 0, (16 + kNoiseStripes) * sizeof(unsigned long long), s));
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.host_scratch);
 
     // cell ids (spatial_index.h:144-165), offset by the job's key base; in
@@ -580,7 +582,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                           cmax, labels, &dstats[1], s);
     tm.mark(M_BORDER0);
     launch_border<T, D>(a, core, box_union ? bcluster : nullptr, rootslot, cmin, cmax, labels,
-                        &dstats[2], amb, s);
+                        &dstats[2], amb, noise_stripes, s);
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
     HPD_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

@@ -514,6 +514,25 @@ __global__ __launch_bounds__(kThreads) void k_border(const DbscanArgs<T> a,
 // lane's current one, or larger while the lane has not yet met two
 // clusters; the smallest adjacent cluster and the two-cluster flag come out
 // exact.
+// Noise rows of the wave, added to one of kNoiseStripes counters (picked by
+// the wave's index): one counter per wave would serialise the atomics of
+// all waves with noise on one address.  k_sum_stripes adds them up.
+__device__ __forceinline__ void count_noise_rows(bool is_noise, unsigned long long* noise) {
+    if (!noise) return;
+    const uint64_t nz = ballot(is_noise);
+    if (lane_id() == 0 && nz) {
+        const uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+        atomicAdd(&noise[w & (kNoiseStripes - 1)], static_cast<unsigned long long>(__popcll(nz)));
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sum_stripes(const unsigned long long* __restrict__ v,
+                                                          unsigned long long* __restrict__ out) {
+    unsigned long long t = 0;
+    for (int i = threadIdx.x; i < kNoiseStripes; i += kThreads) t += v[i];
+    block_add<kThreads>(t, out);
+}
+
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T> a,
                                                            const uint8_t* __restrict__ core,
@@ -521,7 +540,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T
                                                            const uint32_t* __restrict__ order,
                                                            int64_t* __restrict__ labels,
                                                            unsigned long long* __restrict__ stats,
-                                                           int2* __restrict__ amb_list) {
+                                                           int2* __restrict__ amb_list,
+                                                           unsigned long long* __restrict__ noise) {
     Chunk c;
     if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c)) return;
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
@@ -562,6 +582,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T
         return false;
     });
     if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
+    count_noise_rows(active && best == INT_MAX, noise);
     const bool is_amb = active && amb;
     const uint64_t amb_mask = ballot(is_amb);
     const unsigned long long namb = __popcll(amb_mask);
@@ -585,7 +606,8 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
                                                             const int32_t* __restrict__ bcluster,
                                                             int64_t* __restrict__ labels,
                                                             unsigned long long* __restrict__ stats,
-                                                            int2* __restrict__ amb_list) {
+                                                            int2* __restrict__ amb_list,
+                                                            unsigned long long* __restrict__ noise) {
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     bool active = false, amb = false;
     int32_t best = INT_MAX, p = 0, u = 0;
@@ -629,6 +651,7 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
         labels[a.inrow[p]] =
             best == INT_MAX ? 0 : static_cast<int64_t>(best) - jp.row_begin + 1;
     }
+    count_noise_rows(active && best == INT_MAX, noise);
     const bool is_amb = active && amb;
     const uint64_t amb_mask = ballot(is_amb);
     const unsigned long long namb = __popcll(amb_mask);
@@ -910,7 +933,8 @@ void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32
 template <typename T, int D>
 void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                    const int32_t* rootslot, const int32_t* cmin, const int32_t* cmax,
-                   int64_t* labels, unsigned long long* stats, int2* amb_list, hipStream_t s) {
+                   int64_t* labels, unsigned long long* stats, int2* amb_list,
+                   unsigned long long* noise_stripes, hipStream_t s) {
     if (a.n <= 0) return;
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     if constexpr (std::is_floating_point<T>::value) {
@@ -918,13 +942,18 @@ void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
             if (a.nditems > 0)
                 hipLaunchKernelGGL((k_border_boxes<T, D>),
                                    dim3(blocks_for(a.nditems, kWavesPerBlock)), dim3(kThreads), 0,
-                                   s, a, core, bcluster, a.inrow, labels, stats, amb_list);
+                                   s, a, core, bcluster, a.inrow, labels, stats, amb_list,
+                                   noise_stripes);
             if (a.nsitems > 0)
                 hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, a, core, bcluster, labels, stats,
-                                   amb_list);
+                                   amb_list, noise_stripes);
+            if (noise_stripes)
+                hipLaunchKernelGGL(k_sum_stripes, dim3(1), dim3(kThreads), 0, s,
+                                   static_cast<const unsigned long long*>(noise_stripes), &stats[0]);
+            else
+                count_noise(labels, a.n, &stats[0], s);
             HPD_HIP(hipGetLastError());
-            count_noise(labels, a.n, &stats[0], s);
             return;
         }
     }
@@ -970,7 +999,8 @@ void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const
                                             int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t); \
     template void launch_border<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \
                                       const int32_t*, const int32_t*, const int32_t*, int64_t*,   \
-                                      unsigned long long*, int2*, hipStream_t);                   \
+                                      unsigned long long*, int2*, unsigned long long*,            \
+                                      hipStream_t);                                               \
     template void launch_one_thread_mark<T, D>(const DbscanArgs<T>&, const uint8_t*,              \
                                                const int32_t*, const int2*, int32_t, uint8_t*,    \
                                                int32_t*, hipStream_t);                            \

@@ -82,11 +82,15 @@ void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32
 // Border and noise labels of the non-core rows (in box mode box by box:
 // all cores of a box share its cluster).  stats[0] += noise rows,
 // stats[1] += ambiguous border rows; when amb_list is given it receives
-// (slot, cell) of each ambiguous border row (capacity n).
+// (slot, cell) of each ambiguous border row (capacity n).  noise_stripes
+// (kNoiseStripes zeroed counters, or null) lets the box kernels count the
+// noise rows as they label them instead of a pass over the labels.
+constexpr int kNoiseStripes = 1024;
 template <typename T, int D>
 void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                    const int32_t* rootslot, const int32_t* cmin, const int32_t* cmax,
-                   int64_t* labels, unsigned long long* stats, int2* amb_list, hipStream_t s);
+                   int64_t* labels, unsigned long long* stats, int2* amb_list,
+                   unsigned long long* noise_stripes, hipStream_t s);
 
 // The reference's one-thread border choice for the namb ambiguous rows:
 // mark the touched clusters (involved, indexed by cluster position, zeroed
