@@ -79,9 +79,11 @@ __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int3
 // The boxes of chunk c's neighbour rows (for_each_row_box).
 template <typename T, int D, typename Fn>
 __device__ __forceinline__ void for_each_nbr_box(const DbscanArgs<T>& a, const Chunk& c,
-                                                 const int32_t* __restrict__ tag, Fn&& fn) {
+                                                 const int32_t* __restrict__ tag, Fn&& fn,
+                                                 bool own_cell_first = false) {
     for_each_row_box<T, D>(c.r0, c.r1, uniform(a.rcenter[c.u]), a.rcells, a.cbox, a.bstart,
-                           static_cast<const OrdKey<T>*>(a.bbox), tag, fn);
+                           static_cast<const OrdKey<T>*>(a.bbox), tag, fn, INT_MAX,
+                           own_cell_first ? uniform(a.cbox[c.u]) : -1);
 }
 
 __device__ __forceinline__ int32_t wave_min(int32_t v) {
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
             }
             need = valid && cnt < min_pts;
             return !__any(need);
-        });
+        }, true);
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;

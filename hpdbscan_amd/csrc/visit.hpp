@@ -154,7 +154,9 @@ __device__ __forceinline__ void point_box_lh(const double* lo, const double* hi,
 // dependent memory access of its own.  fn(b2, lo, hi, q0, q1, tag) gets the
 // box, its bounding box (double), its rows [q0, q1) and tag[b2] (0 without
 // tag), all wave-uniform; it returns true to stop the visit.  Only boxes
-// below bmax are visited.
+// below bmax are visited.  With first >= 0 (a box of the own row), the own
+// row's boxes are visited from `first` on, wrapping round to the row's
+// earlier boxes: a visit that stops early sees the own cell's boxes first.
 template <typename T, int D, typename Fn>
 __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t center,
                                                  const int2* __restrict__ rcells,
@@ -162,7 +164,7 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
                                                  const int32_t* __restrict__ bstart,
                                                  const OrdKey<T>* __restrict__ bbox,
                                                  const int32_t* __restrict__ tag, Fn&& fn,
-                                                 int32_t bmax = INT_MAX) {
+                                                 int32_t bmax = INT_MAX, int32_t first = -1) {
     const int32_t nrows = r1 - r0;
     const int lane = __lane_id();
     for (int32_t g0 = 0; g0 < nrows; g0 += 64) {
@@ -183,11 +185,20 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
         const int32_t excl = incl - rcnt;
         const int32_t total = bcast(incl, 63);
         const int32_t grows = min(64, nrows - g0);
+        // rotation of the own row (row 0 of the first group, lane 0)
+        int32_t rot = 0, rot_n = 0;
+        if (g0 == 0 && first >= 0) {
+            const int32_t b0 = bcast(rb0, 0);
+            rot_n = bcast(rcnt, 0);
+            rot = (first >= b0 && first < b0 + rot_n) ? first - b0 : 0;
+        }
         for (int32_t k0 = 0; k0 < total; k0 += 64) {
             const int32_t kk = min(k0 + lane, total - 1);
             int32_t row = 0;  // last row whose first box index is <= kk
             for (int32_t t = 1; t < grows; ++t) row = bcast(excl, t) <= kk ? t : row;
-            const int32_t bl = __shfl(rb0, row, 64) + (kk - __shfl(excl, row, 64));
+            int32_t local = kk - __shfl(excl, row, 64);
+            if (row == 0 && rot > 0) local = local + rot < rot_n ? local + rot : local + rot - rot_n;
+            const int32_t bl = __shfl(rb0, row, 64) + local;
             double lol[D], hil[D];
 #pragma unroll
             for (int j = 0; j < D; ++j) {
