@@ -229,8 +229,9 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
     const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
     const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
-    // the cell's own row first: the box's nearest neighbours are there, so
-    // the first hit comes soonest; boxes before b only
+    // the cell's own row first, from the own cell's first box: the box's
+    // nearest neighbours are there, so the first hit comes soonest; boxes
+    // before b only
     for_each_row_box<T, D>(
         r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, bmincore,
         [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
             if (lane_id() == 0) bparent[b] = b2;
             return true;
         },
-        b);
+        b, uniform(cbox[u]));
 }
 
 // Root join: every root of the hook forest (a box without an earlier
@@ -472,7 +473,12 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
         const int32_t e0 = cbox[cs.x];
         if (e0 >= b) continue;  // only boxes before b
         const int32_t e1 = min(cbox[cs.y], b);
-        for (int32_t b2 = e0; b2 < e1; ++b2) {
+        // the own row from the own cell's first box on, wrapping round
+        const int32_t len = e1 - e0;
+        const int32_t own = cbox[u];
+        const int32_t rot = (k < 0 && own > e0 && own < e1) ? own - e0 : 0;
+        for (int32_t t = 0; t < len; ++t) {
+            const int32_t b2 = e0 + (t + rot < len ? t + rot : t + rot - len);
             if (bmincore[b2] == INT_MAX) continue;
             // the two bounding boxes: beyond eps of each other, nothing to
             // test; every pair within eps, the boxes' cores meet
