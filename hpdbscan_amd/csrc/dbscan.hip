@@ -230,10 +230,19 @@ __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
                 const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
                 T pc[D];
                 load_row<T, D>(a.soa, a.ld, p, pc);
-                for (uint32_t r = a.roff[u]; r < a.roff[u + 1] && cnt < min_pts; ++r) {
+                // the own row first, from the own cell's first box on
+                const int32_t r0 = static_cast<int32_t>(a.roff[u]);
+                const int32_t nr = static_cast<int32_t>(a.roff[u + 1]) - r0;
+                const int32_t center = a.rcenter[u];
+                const int32_t own = a.cbox[u];
+                for (int32_t k = -1; k < nr && cnt < min_pts; ++k) {
+                    const int32_t r = k < 0 ? center : r0 + k;
+                    if (k >= 0 && r == center) continue;
                     const int2 cs = a.rcells[r];
-                    const int32_t e1 = a.cbox[cs.y];
-                    for (int32_t b2 = a.cbox[cs.x]; b2 < e1 && cnt < min_pts; ++b2) {
+                    const int32_t e0 = a.cbox[cs.x], len = a.cbox[cs.y] - e0;
+                    const int32_t rot = (k < 0 && own > e0 && own < e0 + len) ? own - e0 : 0;
+                    for (int32_t t = 0; t < len && cnt < min_pts; ++t) {
+                        const int32_t b2 = e0 + (t + rot < len ? t + rot : t + rot - len);
                         double gmin2, gmax2;
                         point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
                         if (gmin2 > gap2_max) continue;
