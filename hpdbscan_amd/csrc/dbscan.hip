@@ -591,7 +591,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T
             best = min(best, cl);
         }
         return false;
-    });
+    }, true);
     if (active) labels[order[p]] = best == INT_MAX ? 0 : static_cast<int64_t>(best) - first_pos + 1;
     count_noise_rows(active && best == INT_MAX, noise);
     const bool is_amb = active && amb;
@@ -636,10 +636,20 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
         const OrdKey<T>* bbox = static_cast<const OrdKey<T>*>(a.bbox);
         T pc[D];
         load_row<T, D>(a.soa, a.ld, p, pc);
-        for (uint32_t r = a.roff[u]; r < a.roff[u + 1]; ++r) {
+        // the own row first, from the own cell's first box on: the own
+        // cluster is found first and its other boxes are skipped
+        const int32_t r0 = static_cast<int32_t>(a.roff[u]);
+        const int32_t nr = static_cast<int32_t>(a.roff[u + 1]) - r0;
+        const int32_t center = a.rcenter[u];
+        const int32_t own = a.cbox[u];
+        for (int32_t k = -1; k < nr; ++k) {
+            const int32_t r = k < 0 ? center : r0 + k;
+            if (k >= 0 && r == center) continue;
             const int2 cs = a.rcells[r];
-            const int32_t e1 = a.cbox[cs.y];
-            for (int32_t b2 = a.cbox[cs.x]; b2 < e1; ++b2) {
+            const int32_t e0 = a.cbox[cs.x], len = a.cbox[cs.y] - e0;
+            const int32_t rot = (k < 0 && own > e0 && own < e0 + len) ? own - e0 : 0;
+            for (int32_t t = 0; t < len; ++t) {
+                const int32_t b2 = e0 + (t + rot < len ? t + rot : t + rot - len);
                 const int32_t cl = bcluster[b2];
                 if (cl < 0 || cl == best || (cl > best && amb)) continue;
                 double gmin2, gmax2;
