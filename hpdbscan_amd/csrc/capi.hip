@@ -470,6 +470,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     a.rcenter = rcenter;
     a.m = m;
     a.jobs = jobs;
+    a.njobs = njobs;
     a.job_of_cell = job_of_cell;
     a.pos_of_slot = nullptr;  // slots are positions unless boxes reorder them
     a.inrow = inrow_of_slot;
@@ -578,8 +579,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     tm.mark(M_UNION);
     const bool one_thread = ctx.border_rule == HPD_BORDER_ONE_THREAD;
     int2* amb = one_thread ? A.take_n<int2>(nn) : nullptr;
-    launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin, rootslot, cmin,
-                          cmax, labels, &dstats[1], s);
+    // box mode reads rootslot only under the one-thread border rule
+    launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin,
+                          box_union && !one_thread ? nullptr : rootslot, cmin, cmax, labels,
+                          &dstats[1], s);
     tm.mark(M_BORDER0);
     launch_border<T, D>(a, core, box_union ? bcluster : nullptr, rootslot, cmin, cmax, labels,
                         &dstats[2], amb, noise_stripes, s);

@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
     const uint8_t* __restrict__ core, const int32_t* __restrict__ box_of_slot,
     const int32_t* __restrict__ bcell, const int32_t* __restrict__ bcluster,
     const uint32_t* __restrict__ inrow, int64_t* __restrict__ labels,
-    int32_t* __restrict__ rootslot, unsigned long long* __restrict__ ncore) {
+    int32_t* __restrict__ rootslot, unsigned long long* __restrict__ ncore, bool one_job) {
     // grid-stride over at most kLabelBlocks blocks, so that the core count
     // of the run statistics (folded in here) takes few atomics
     unsigned long long cores = 0;
@@ -397,8 +397,9 @@ __global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
         ++cores;
         const int32_t b = box_of_slot[p];
         const int32_t r = bcluster[b];
-        rootslot[p] = r;
-        const int64_t first_pos = jobs[job_of_cell[bcell[b]]].row_begin;
+        if (rootslot) rootslot[p] = r;
+        // one job starts at row 0
+        const int64_t first_pos = one_job ? 0 : jobs[job_of_cell[bcell[b]]].row_begin;
         labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
     }
     if (ncore) block_add<kThreads>(cores, ncore);
@@ -937,7 +938,7 @@ void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32
         const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
         hipLaunchKernelGGL(k_label_cores_boxes<T>, dim3(grid), dim3(kThreads), 0, s, a.n, a.jobs,
                            a.job_of_cell, core, a.box_of_slot, a.bcell, bcluster, a.inrow, labels,
-                           rootslot, ncore);
+                           rootslot, ncore, a.njobs == 1);
         HPD_HIP(hipGetLastError());
         return;
     }

@@ -23,6 +23,7 @@ struct DbscanArgs {
     const int32_t* rcenter = nullptr; // index into ranges of each cell's own row
     int32_t m = 0;                   // occupied cells
     const JobParams<T>* jobs = nullptr;      // eps2, minPts, first row per job
+    int32_t njobs = 1;
     const int32_t* job_of_cell = nullptr;    // job of every occupied cell
     // slot order: rows sorted by cell, then (when boxes are used) by sub-box
     const uint32_t* pos_of_slot = nullptr;  // sorted position per slot (null: identity)
@@ -73,7 +74,8 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 // rootslot[slot] = cluster, cmin / cmax = smallest / largest cluster per
 // occupied cell (not in box mode, where the border pass reads bcluster).
 // Labels go to input order through a.inrow; *ncore += number of cores.
-// rootslot: ld entries.
+// rootslot: ld entries (may be null in box mode, where only the one-thread
+// border rule reads it).
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
