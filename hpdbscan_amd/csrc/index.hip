@@ -87,8 +87,9 @@ __device__ __forceinline__ void decode_cell(uint64_t key, const CellParams& cp, 
 }
 
 // Visit the neighbour rows of a cell in ascending cell-id order.
-// fn(row_lo_key, row_hi_key, is_center) gets the inclusive cell-id bounds of
-// each row that lies inside the grid.  Odometer order over the offsets of
+// fn(row_lo_key, row_hi_key, is_center, nz) gets the inclusive cell-id
+// bounds of each row that lies inside the grid and the number of dimensions
+// 1..D-1 in which the row is offset from the own cell (0 for the own row).  Odometer order over the offsets of
 // swapped dimensions 1..D-1, most significant dimension slowest, is ascending
 // because the offsets of the lower dimensions never reach one stride of the
 // next.
@@ -103,7 +104,7 @@ __device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp,
     const int center_row = (nrows - 1) / 2;  // every digit 1: offset 0 in all dims
     for (int r = 0; r < nrows; ++r) {
         // digit k-1 of r in base 3 is the offset (0 -> -1, 1 -> 0, 2 -> +1)
-        int rem = r;
+        int rem = r, nz = 0;
         bool inside = true;
         uint64_t base = key;
         for (int k = 1; k < cp.d; ++k) {
@@ -112,13 +113,15 @@ __device__ __forceinline__ void for_each_row(uint64_t key, const CellParams& cp,
             if (dgt == 0) {
                 if (idx[k] == 0) inside = false;
                 base -= cp.stride[k];
+                ++nz;
             } else if (dgt == 2) {
                 if (idx[k] + 1 >= cp.ncell[k]) inside = false;
                 base += cp.stride[k];
+                ++nz;
             }
         }
         if (!inside) continue;
-        fn(base - lo0, base + hi0, r == center_row);
+        fn(base - lo0, base + hi0, r == center_row, nz);
     }
 }
 
@@ -149,7 +152,8 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              unsigned long long* __restrict__ sumcand,
                                                              int2* __restrict__ stash,
                                                              int32_t stash_rows,
-                                                             int32_t* __restrict__ center_local) {
+                                                             int32_t* __restrict__ center_local,
+                                                             uint64_t* __restrict__ row_class) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
@@ -160,14 +164,16 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         const uint64_t base = jobs[j].key_base;
         uint32_t cnt = 0;
         int32_t from = 0, center = 0;
+        uint64_t classes = 0;  // 2 bits per found row (up to 27 rows, D <= 4)
         int2* mine = stash ? stash + static_cast<int64_t>(u) * stash_rows : nullptr;
-        for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool is_center) {
+        for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool is_center, int nz) {
             const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
             from = ab.y;
             if (ab.y > ab.x) {
                 if (mine) {
                     if (is_center) center = static_cast<int32_t>(cnt);
                     mine[cnt] = ab;
+                    if (row_class) classes |= static_cast<uint64_t>(nz & 3) << (2 * cnt);
                 }
                 ++cnt;
                 cand += static_cast<uint64_t>(ustart[ab.y] - ustart[ab.x]);
@@ -175,6 +181,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         });
         nranges[u] = cnt;
         if (mine) center_local[u] = center;
+        if (row_class) row_class[u] = classes;
         count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
         // the reference queries a row only with >= minPts candidates
         // (hpdbscan.h:74); with fewer, no row of the cell can be core
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
     int32_t center = static_cast<int32_t>(w);
     int32_t from = 0;
     for_each_row(static_cast<uint64_t>(ucell[u]) - base, jobs[j].cp,
-                 [&](uint64_t lo, uint64_t hi, bool is_center) {
+                 [&](uint64_t lo, uint64_t hi, bool is_center, int) {
                      const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
                      from = ab.y;
                      if (ab.y > ab.x) {
@@ -225,17 +232,36 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_copy(const int2* __restri
                                                             const uint32_t* __restrict__ roff,
                                                             int2* __restrict__ ranges,
                                                             int2* __restrict__ rcells,
-                                                            int32_t* rcenter) {
+                                                            int32_t* rcenter,
+                                                            const uint64_t* __restrict__ row_class) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const uint32_t w0 = roff[u], cnt = roff[u + 1] - w0;
     const int2* mine = stash + static_cast<int64_t>(u) * stash_rows;
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const int2 ab = mine[k];
-        rcells[w0 + k] = ab;
-        ranges[w0 + k] = make_int2(ustart[ab.x], ustart[ab.y]);
+    if (!row_class) {
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const int2 ab = mine[k];
+            rcells[w0 + k] = ab;
+            ranges[w0 + k] = make_int2(ustart[ab.x], ustart[ab.y]);
+        }
+        rcenter[u] = static_cast<int32_t>(w0) + rcenter[u];  // local index from the count pass
+        return;
     }
-    rcenter[u] = static_cast<int32_t>(w0) + rcenter[u];  // local index from the count pass
+    // nearest rows first: the own row, then rows offset in one dimension,
+    // then in two, then in three (a visit that stops early sees the rows
+    // with the most within-eps neighbours first)
+    const uint64_t classes = row_class[u];
+    uint32_t w = w0;
+    for (uint32_t c = 0; c < 4; ++c) {
+        for (uint32_t k = 0; k < cnt; ++k) {
+            if (((classes >> (2 * k)) & 3) != c) continue;
+            const int2 ab = mine[k];
+            rcells[w] = ab;
+            ranges[w] = make_int2(ustart[ab.x], ustart[ab.y]);
+            ++w;
+        }
+    }
+    rcenter[u] = static_cast<int32_t>(w0);  // the own row is the only one of class 0
 }
 
 __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
@@ -338,18 +364,19 @@ template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           int2* stash, int32_t stash_rows, int32_t* rcenter, hipStream_t s) {
+                           int2* stash, int32_t stash_rows, int32_t* rcenter, uint64_t* row_class,
+                           hipStream_t s) {
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
-                       stash, stash_rows, rcenter);
+                       stash, stash_rows, rcenter, row_class);
     HPD_HIP(hipGetLastError());
 }
 
 void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
-                          hipStream_t s) {
+                          const uint64_t* row_class, hipStream_t s) {
     hipLaunchKernelGGL(k_neighbor_rows_copy, dim3(blocks_for(m, 256)), dim3(256), 0, s, stash,
-                       stash_rows, ustart, m, roff, ranges, rcells, rcenter);
+                       stash_rows, ustart, m, roff, ranges, rcells, rcenter, row_class);
     HPD_HIP(hipGetLastError());
 }
 
@@ -384,11 +411,13 @@ HPD_INST_INDEX(uint64_t)
     template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, hipStream_t);    \
+                                                     int2*, int32_t, int32_t*, uint64_t*,       \
+                                                     hipStream_t);                              \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, hipStream_t);    \
+                                                     int2*, int32_t, int32_t*, uint64_t*,       \
+                                                     hipStream_t);                              \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \
