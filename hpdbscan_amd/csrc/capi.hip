@@ -389,15 +389,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* stash = scratch64 && static_cast<uint64_t>(m) * static_cast<uint64_t>(stash_rows) <= nn
                       ? reinterpret_cast<int2*>(scratch64)
                       : nullptr;
-    // up to 27 rows per cell (D <= 4): their offset classes follow the
-    // stash, so that the copy can put the nearest rows first
-    uint64_t* row_class =
-        stash && stash_rows <= 27 &&
-                static_cast<uint64_t>(m) * static_cast<uint64_t>(stash_rows + 1) <= nn
-            ? reinterpret_cast<uint64_t*>(stash + static_cast<int64_t>(m) * stash_rows)
-            : nullptr;
+    // up to 27 rows per cell (D <= 4): the copy puts the nearest rows first
+    const bool by_class = stash && D <= 4;
     launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
-                                &dstats[0], stash, stash_rows, rcenter, row_class, s);
+                                &dstats[0], stash, stash_rows, rcenter, by_class, s);
     launch_chunk_counts(ustart, m, nchunks, s);
     // box mode: dense-cell chunks and sparse-cell rows as separate work
     // lists; their per-cell counts live in the (now unused) run flags
@@ -455,7 +450,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* citems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
     if (stash)
         launch_neighbor_copy(stash, stash_rows, ustart, m, nranges, ranges, rcells, rcenter,
-                             row_class, s);
+                             by_class, s);
     else
         launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
                                    rcenter, s);

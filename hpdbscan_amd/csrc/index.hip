@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              int2* __restrict__ stash,
                                                              int32_t stash_rows,
                                                              int32_t* __restrict__ center_local,
-                                                             uint64_t* __restrict__ row_class) {
+                                                             bool by_class) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
@@ -164,7 +164,6 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         const uint64_t base = jobs[j].key_base;
         uint32_t cnt = 0;
         int32_t from = 0, center = 0;
-        uint64_t classes = 0;  // 2 bits per found row (up to 27 rows, D <= 4)
         int2* mine = stash ? stash + static_cast<int64_t>(u) * stash_rows : nullptr;
         for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool is_center, int nz) {
             const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
@@ -172,8 +171,15 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
             if (ab.y > ab.x) {
                 if (mine) {
                     if (is_center) center = static_cast<int32_t>(cnt);
-                    mine[cnt] = ab;
-                    if (row_class) classes |= static_cast<uint64_t>(nz & 3) << (2 * cnt);
+                    // with by_class, the row's offset class (nz <= 3) rides in
+                    // the top bits of its cell indices (both below 2^31)
+                    mine[cnt] = by_class ? make_int2(static_cast<int32_t>(
+                                                         static_cast<uint32_t>(ab.x) |
+                                                         (static_cast<uint32_t>(nz & 1) << 31)),
+                                                     static_cast<int32_t>(
+                                                         static_cast<uint32_t>(ab.y) |
+                                                         (static_cast<uint32_t>((nz >> 1) & 1) << 31)))
+                                         : ab;
                 }
                 ++cnt;
                 cand += static_cast<uint64_t>(ustart[ab.y] - ustart[ab.x]);
@@ -181,7 +187,6 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         });
         nranges[u] = cnt;
         if (mine) center_local[u] = center;
-        if (row_class) row_class[u] = classes;
         count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
         // the reference queries a row only with >= minPts candidates
         // (hpdbscan.h:74); with fewer, no row of the cell can be core
@@ -232,13 +237,12 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_copy(const int2* __restri
                                                             const uint32_t* __restrict__ roff,
                                                             int2* __restrict__ ranges,
                                                             int2* __restrict__ rcells,
-                                                            int32_t* rcenter,
-                                                            const uint64_t* __restrict__ row_class) {
+                                                            int32_t* rcenter, bool by_class) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const uint32_t w0 = roff[u], cnt = roff[u + 1] - w0;
     const int2* mine = stash + static_cast<int64_t>(u) * stash_rows;
-    if (!row_class) {
+    if (!by_class) {
         for (uint32_t k = 0; k < cnt; ++k) {
             const int2 ab = mine[k];
             rcells[w0 + k] = ab;
@@ -250,12 +254,14 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_copy(const int2* __restri
     // nearest rows first: the own row, then rows offset in one dimension,
     // then in two, then in three (a visit that stops early sees the rows
     // with the most within-eps neighbours first)
-    const uint64_t classes = row_class[u];
     uint32_t w = w0;
     for (uint32_t c = 0; c < 4; ++c) {
         for (uint32_t k = 0; k < cnt; ++k) {
-            if (((classes >> (2 * k)) & 3) != c) continue;
-            const int2 ab = mine[k];
+            const int2 e = mine[k];
+            const uint32_t x = static_cast<uint32_t>(e.x), y = static_cast<uint32_t>(e.y);
+            if (((x >> 31) | ((y >> 31) << 1)) != c) continue;
+            const int2 ab = make_int2(static_cast<int32_t>(x & 0x7fffffffu),
+                                      static_cast<int32_t>(y & 0x7fffffffu));
             rcells[w] = ab;
             ranges[w] = make_int2(ustart[ab.x], ustart[ab.y]);
             ++w;
@@ -364,19 +370,19 @@ template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           int2* stash, int32_t stash_rows, int32_t* rcenter, uint64_t* row_class,
+                           int2* stash, int32_t stash_rows, int32_t* rcenter, bool by_class,
                            hipStream_t s) {
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
-                       stash, stash_rows, rcenter, row_class);
+                       stash, stash_rows, rcenter, by_class);
     HPD_HIP(hipGetLastError());
 }
 
 void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
-                          const uint64_t* row_class, hipStream_t s) {
+                          bool by_class, hipStream_t s) {
     hipLaunchKernelGGL(k_neighbor_rows_copy, dim3(blocks_for(m, 256)), dim3(256), 0, s, stash,
-                       stash_rows, ustart, m, roff, ranges, rcells, rcenter, row_class);
+                       stash_rows, ustart, m, roff, ranges, rcells, rcenter, by_class);
     HPD_HIP(hipGetLastError());
 }
 
@@ -411,12 +417,12 @@ HPD_INST_INDEX(uint64_t)
     template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, uint64_t*,       \
+                                                     int2*, int32_t, int32_t*, bool,            \
                                                      hipStream_t);                              \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, uint64_t*,       \
+                                                     int2*, int32_t, int32_t*, bool,            \
                                                      hipStream_t);                              \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
                                                     const JobParams<T>*, const int32_t*,        \

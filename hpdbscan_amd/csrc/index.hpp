@@ -22,21 +22,22 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 // Σ_u count(u) * |candidates(u)| to *sumcand.  With a stash (m * stash_rows
 // entries, stash_rows >= the most neighbour rows a cell can have) the
 // rows' cell ranges go to stash[u * stash_rows ..] and the own row's index
-// among them to rcenter[u], for launch_neighbor_copy; with row_class (m
-// entries; stash_rows <= 27) also the rows' offset classes, 2 bits each.
+// among them to rcenter[u], for launch_neighbor_copy; with by_class (D <= 4)
+// each row's offset class (the number of dimensions it is offset in) rides
+// in the top bits of its two cell indices.
 template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           int2* stash, int32_t stash_rows, int32_t* rcenter, uint64_t* row_class,
+                           int2* stash, int32_t stash_rows, int32_t* rcenter, bool by_class,
                            hipStream_t s);
 
 // launch_neighbor_fill's output from the count pass's stash (roff scanned);
-// with row_class the rows of each cell go nearest first (own row, then by
+// with by_class the rows of each cell go nearest first (own row, then by
 // the number of offset dimensions) instead of ascending
 void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
-                          const uint64_t* row_class, hipStream_t s);
+                          bool by_class, hipStream_t s);
 
 // ranges[roff[u] ..) = [first row, end row) of each non-empty neighbour row,
 // ascending; rcells[..] = [first, end) occupied-cell index of the same row;
