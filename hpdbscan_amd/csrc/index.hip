@@ -270,6 +270,49 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_copy(const int2* __restri
     rcenter[u] = static_cast<int32_t>(w0);  // the own row is the only one of class 0
 }
 
+// The by-class copy with at most R rows per cell (R = 3^(D-1) <= 27): the
+// rows are read once into registers, counted per class, and written at
+// their class's place.
+template <int R>
+__global__ __launch_bounds__(256) void k_neighbor_rows_copy_class(const int2* __restrict__ stash,
+                                                                  const int32_t* __restrict__ ustart,
+                                                                  int32_t m,
+                                                                  const uint32_t* __restrict__ roff,
+                                                                  int2* __restrict__ ranges,
+                                                                  int2* __restrict__ rcells,
+                                                                  int32_t* __restrict__ rcenter) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u >= m) return;
+    const uint32_t w0 = roff[u], cnt = roff[u + 1] - w0;
+    const int2* mine = stash + static_cast<int64_t>(u) * R;
+    int2 e[R];
+    uint32_t cls[R];
+    uint32_t n1 = 0, n2 = 0;  // rows of class 1 and 2 (class 0 is the own row alone)
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        cls[k] = 4;
+        if (static_cast<uint32_t>(k) < cnt) {
+            const int2 v = mine[k];
+            const uint32_t x = static_cast<uint32_t>(v.x), y = static_cast<uint32_t>(v.y);
+            cls[k] = (x >> 31) | ((y >> 31) << 1);
+            e[k] = make_int2(static_cast<int32_t>(x & 0x7fffffffu),
+                             static_cast<int32_t>(y & 0x7fffffffu));
+            n1 += cls[k] == 1 ? 1u : 0u;
+            n2 += cls[k] == 2 ? 1u : 0u;
+        }
+    }
+    uint32_t p0 = w0, p1 = w0 + 1, p2 = w0 + 1 + n1, p3 = w0 + 1 + n1 + n2;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (cls[k] > 3) continue;
+        const uint32_t c = cls[k];
+        const uint32_t w = c == 0 ? p0++ : c == 1 ? p1++ : c == 2 ? p2++ : p3++;
+        rcells[w] = e[k];
+        ranges[w] = make_int2(ustart[e[k].x], ustart[e[k].y]);
+    }
+    rcenter[u] = static_cast<int32_t>(w0);  // the own row is the only one of class 0
+}
+
 __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
                                                       uint32_t* __restrict__ nchunks) {
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
@@ -381,8 +424,16 @@ void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
 void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* ustart, int32_t m,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
                           bool by_class, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbor_rows_copy, dim3(blocks_for(m, 256)), dim3(256), 0, s, stash,
-                       stash_rows, ustart, m, roff, ranges, rcells, rcenter, by_class);
+    const dim3 grid(blocks_for(m, 256));
+    if (by_class && stash_rows == 9)
+        hipLaunchKernelGGL(k_neighbor_rows_copy_class<9>, grid, dim3(256), 0, s, stash, ustart, m,
+                           roff, ranges, rcells, rcenter);
+    else if (by_class && stash_rows == 27)
+        hipLaunchKernelGGL(k_neighbor_rows_copy_class<27>, grid, dim3(256), 0, s, stash, ustart, m,
+                           roff, ranges, rcells, rcenter);
+    else
+        hipLaunchKernelGGL(k_neighbor_rows_copy, grid, dim3(256), 0, s, stash, stash_rows, ustart, m,
+                           roff, ranges, rcells, rcenter, by_class);
     HPD_HIP(hipGetLastError());
 }
 
