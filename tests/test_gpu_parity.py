@@ -183,6 +183,34 @@ def test_box_union_used_and_exact(ctx):
     assert_parity(x, 0.3, 8, got, oracle.cluster(x, 0.3, 8))
 
 
+def dense_blobs(n, d, seed):
+    """Four tight blobs on separated centres with a little noise around them:
+    few occupied cells, each holding many rows."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((4, d))
+    for i in range(4):
+        c[i, i % d] = 1.0 + i // d
+    nb = int(n * 0.97)
+    x = np.concatenate([c[rng.integers(0, 4, nb)] + rng.normal(0, 0.05, (nb, d)),
+                        c[rng.integers(0, 4, n - nb)] + rng.uniform(-0.25, 0.25, (n - nb, d))])
+    return x[rng.permutation(n)].astype(np.float32)
+
+
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_dense_cells_nearest_first_rows(ctx, d):
+    """With at least 3^(D-1) rows per occupied cell the neighbour rows are
+    kept from the count pass and stored nearest first (own row, then by the
+    number of offset dimensions); the labels equal the oracle's."""
+    n = 30000 if d == 2 else 50000
+    x = dense_blobs(n, d, 50 + d)
+    say(f"dense cells d={d} n={n} eps=0.1 minPts=20")
+    got = ctx.cluster(x, 0.1, 20)
+    st = ctx.stats()
+    assert st["occupied_cells"] * 3 ** (d - 1) <= n  # the nearest-first path ran
+    assert_parity(x, 0.1, 20, got, oracle.cluster(x, 0.1, 20))
+    assert len(np.unique(got[got < 0])) == 4 and (got > 0).any() and (got == 0).any()
+
+
 def test_skewed_2d_sample(ctx):
     from hpdbscan_amd import datagen
     x = datagen.skewed_2d(n=200_000, seed=3)
