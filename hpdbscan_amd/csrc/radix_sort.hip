@@ -1,4 +1,4 @@
-// Stable LSD radix sort of (cell id, row index) pairs, 8 bits per pass.
+// Stable LSD radix sort of (cell id, row index) pairs, 8 or 9 bits per pass.
 //
 // Replaces the reference's cell histogram + serial counting-sort scatter
 // (spatial_index.h:144-216).  Per pass:
@@ -10,7 +10,8 @@
 //                      in LDS in digit order, then written out so that
 //                      consecutive threads write consecutive places
 // A tile is 4096 keys: 4 waves x 16 iterations x 64 lanes; the stable order
-// inside a tile is (wave, iteration, lane), i.e. input order.
+// inside a tile is (wave, iteration, lane), i.e. input order.  9-bit digits
+// (512 buckets, two per thread) are used where they save a pass.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -21,16 +22,18 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kIters = 16;
 constexpr int kTile = kThreads * kIters;  // 4096
-constexpr int kBuckets = 256;
+constexpr int kMaxBuckets = 512;          // 9-bit digits
 
 __host__ __device__ inline int64_t tiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
 
-template <typename K>
+template <typename K, int kBits>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ keys, int64_t n,
                                                          int shift, uint32_t* __restrict__ hist,
                                                          int64_t ntiles) {
-    __shared__ uint32_t cnt[kBuckets];
-    cnt[threadIdx.x] = 0;
+    constexpr int B = 1 << kBits;
+    constexpr uint32_t kMask = B - 1;
+    __shared__ uint32_t cnt[B];
+    for (int dg = threadIdx.x; dg < B; dg += kThreads) cnt[dg] = 0;
     __syncthreads();
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
@@ -39,15 +42,16 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ k
     for (int it = 0; it < kIters; ++it) {
         const int64_t idx = base + it * 64 + l;
         if (idx < n) {
-            const uint32_t dgt = static_cast<uint32_t>((keys[idx] >> shift) & 0xff);
+            const uint32_t dgt = static_cast<uint32_t>((keys[idx] >> shift) & kMask);
             atomicAdd(&cnt[dgt], 1u);
         }
     }
     __syncthreads();
-    hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    for (int dg = threadIdx.x; dg < B; dg += kThreads)
+        hist[static_cast<int64_t>(dg) * ntiles + blockIdx.x] = cnt[dg];
 }
 
-template <typename K>
+template <typename K, int kBits>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             K* __restrict__ kout,
@@ -55,17 +59,22 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
                                                             int shift,
                                                             const uint32_t* __restrict__ offs,
                                                             int64_t ntiles) {
-    __shared__ uint32_t cnt[4][kBuckets];
-    __shared__ uint32_t gbase[kBuckets];
-    __shared__ uint32_t tbase[kBuckets];  // digit's first place in the staged tile
+    constexpr int B = 1 << kBits;
+    constexpr uint32_t kMask = B - 1;
+    constexpr int kOwn = B / kThreads;  // consecutive digits per thread
+    __shared__ uint32_t cnt[4][B];
+    __shared__ uint32_t gbase[B];
+    __shared__ uint32_t tbase[B];  // digit's first place in the staged tile
     __shared__ uint32_t wsum[kThreads / 64];
     __shared__ K skey[kTile];
     __shared__ uint32_t sval[kTile];
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
+    for (int dg = threadIdx.x; dg < B; dg += kThreads) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) cnt[i][threadIdx.x] = 0;
-    gbase[threadIdx.x] = offs[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x];
+        for (int i = 0; i < 4; ++i) cnt[i][dg] = 0;
+        gbase[dg] = offs[static_cast<int64_t>(dg) * ntiles + blockIdx.x];
+    }
     __syncthreads();
 
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + w * (kTile / 4);
@@ -79,10 +88,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
         const int64_t idx = base + it * 64 + l;
         const bool valid = idx < n;
         const K k = valid ? kin[idx] : K(0);
-        const uint32_t d = static_cast<uint32_t>((k >> shift) & 0xff);
+        const uint32_t d = static_cast<uint32_t>((k >> shift) & kMask);
         uint64_t peers = ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kBits; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = ballot(bit);
             peers &= bit ? m : ~m;
@@ -97,13 +106,21 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     }
     __syncthreads();
     // exclusive prefix of each digit over the four waves, and of the digits'
-    // tile totals over the digits (thread t owns digit t)
+    // tile totals over the digits (thread t owns digits [t * kOwn, ..))
+    uint32_t tot[kOwn];
     uint32_t run = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t c = cnt[i][threadIdx.x];
-        cnt[i][threadIdx.x] = run;
-        run += c;
+    for (int o = 0; o < kOwn; ++o) {
+        const int dg = threadIdx.x * kOwn + o;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t c = cnt[i][dg];
+            cnt[i][dg] = acc;
+            acc += c;
+        }
+        tot[o] = acc;
+        run += acc;
     }
     uint32_t inc = run;
 #pragma unroll
@@ -117,7 +134,12 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
 #pragma unroll
     for (int i = 0; i < kThreads / 64; ++i)
         if (i < w) woff += wsum[i];
-    tbase[threadIdx.x] = woff + inc - run;
+    uint32_t first = woff + inc - run;
+#pragma unroll
+    for (int o = 0; o < kOwn; ++o) {
+        tbase[threadIdx.x * kOwn + o] = first;
+        first += tot[o];
+    }
     __syncthreads();
     // stage the tile in digit order (stable: waves, iterations and lanes are
     // in input order) ...
@@ -139,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
         const int at = it * kThreads + threadIdx.x;
         if (at < valid_in_tile) {
             const K k = skey[at];
-            const uint32_t d = static_cast<uint32_t>((k >> shift) & 0xff);
+            const uint32_t d = static_cast<uint32_t>((k >> shift) & kMask);
             const uint32_t dst = gbase[d] + (static_cast<uint32_t>(at) - tbase[d]);
             kout[dst] = k;
             vout[dst] = sval[at];
@@ -147,29 +169,22 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     }
 }
 
-}  // namespace
-
-size_t radix_hist_words(int64_t n) {
-    return static_cast<size_t>(kBuckets) * static_cast<size_t>(tiles_of(n > 0 ? n : 1));
-}
-
-template <typename K>
-bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
-                      int bits, uint32_t* hist, uint32_t* tmp, hipStream_t s) {
-    if (n <= 1 || bits <= 0) return false;
+template <typename K, int kBits>
+bool sort_passes(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n, int bits,
+                 uint32_t* hist, uint32_t* tmp, hipStream_t s) {
     const int64_t nt = tiles_of(n);
-    const int64_t nh = static_cast<int64_t>(kBuckets) * nt;
+    const int64_t nh = static_cast<int64_t>(1 << kBits) * nt;
     K* kin = keys_a;
     uint32_t* vin = vals_a;
     K* kout = keys_b;
     uint32_t* vout = vals_b;
     bool in_b = false;
-    for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(static_cast<unsigned>(nt)), dim3(kThreads), 0, s,
-                           kin, n, shift, hist, nt);
+    for (int shift = 0; shift < bits; shift += kBits) {
+        hipLaunchKernelGGL((k_radix_hist<K, kBits>), dim3(static_cast<unsigned>(nt)),
+                           dim3(kThreads), 0, s, kin, n, shift, hist, nt);
         exclusive_scan_u32(hist, hist, nh, tmp, nullptr, s);
-        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(static_cast<unsigned>(nt)), dim3(kThreads), 0,
-                           s, kin, vin, kout, vout, n, shift, hist, nt);
+        hipLaunchKernelGGL((k_radix_scatter<K, kBits>), dim3(static_cast<unsigned>(nt)),
+                           dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift, hist, nt);
         HPD_HIP(hipGetLastError());
         K* tk = kin;
         kin = kout;
@@ -180,6 +195,22 @@ bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
         in_b = !in_b;
     }
     return in_b;
+}
+
+}  // namespace
+
+size_t radix_hist_words(int64_t n) {
+    return static_cast<size_t>(kMaxBuckets) * static_cast<size_t>(tiles_of(n > 0 ? n : 1));
+}
+
+// 9-bit digits where they save a pass (e.g. 25-27 key bits: 3 passes, not 4)
+template <typename K>
+bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                      int bits, uint32_t* hist, uint32_t* tmp, hipStream_t s) {
+    if (n <= 1 || bits <= 0) return false;
+    if ((bits + 8) / 9 < (bits + 7) / 8)
+        return sort_passes<K, 9>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
+    return sort_passes<K, 8>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
 }
 
 template bool radix_sort_pairs<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
