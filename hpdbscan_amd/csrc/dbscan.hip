@@ -58,7 +58,7 @@ struct Chunk {
     int32_t u;       // occupied cell
     int32_t first;   // first row of the chunk
     int32_t end;     // end row of the cell
-    int32_t r0, r1;  // its neighbour ranges (ascending)
+    int32_t r0, r1;  // its neighbour ranges (ascending, or nearest first)
 };
 
 __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int32_t nitems,
@@ -280,8 +280,8 @@ __global__ __launch_bounds__(kThreads) void k_hook(const DbscanArgs<T> a,
     if (!__any(active)) return;
     T pc[D];
     load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
-    // candidates arrive in ascending slot order, so the first hit is the
-    // smallest adjacent core before p
+    // any within-eps core before p is a valid parent (parent[x] < x keeps the
+    // hooks a forest); the first one found is taken
     int32_t best = p;
     bool found = !active;
     auto body = [&](int32_t q, const T* qc, uint8_t cq, int32_t) {
@@ -293,10 +293,12 @@ __global__ __launch_bounds__(kThreads) void k_hook(const DbscanArgs<T> a,
     };
     auto stop = [&] { return static_cast<bool>(__all(found)); };
     const int32_t pmax = uniform(min(c.first + kWave, c.end) - 1);
+    // a cell's rows may be stored nearest first rather than ascending, so a
+    // row that starts at or after pmax is skipped, not taken as the end
     for (int32_t r = c.r0; r < c.r1; ++r) {
         const int2 rg = a.ranges[r];
         const int32_t qb = uniform(rg.x);
-        if (qb >= pmax) break;
+        if (qb >= pmax) continue;
         if (visit_range<T, D>(a.soa, a.ld, core, nullptr, qb, uniform(min(rg.y, pmax)), body, stop))
             break;
     }
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(kThreads) void k_union(const DbscanArgs<T> a,
     for (int32_t r = c.r0; r < c.r1; ++r) {
         const int2 rg = a.ranges[r];
         const int32_t qb = uniform(rg.x);
-        if (qb >= pmax) break;
+        if (qb >= pmax) continue;  // rows are not always ascending (nearest first)
         if (R >= 0 && cells_single_root(a.rcells[r], cmin, cmax, R)) continue;
         visit_range<T, D>(a.soa, a.ld, core, rsnap, qb, uniform(min(rg.y, pmax)), body, never);
     }
@@ -750,7 +752,9 @@ __global__ __launch_bounds__(kThreads) void k_ot_relax(const DbscanArgs<T> a,
     for (int32_t r = c.r0; r < c.r1; ++r) {
         const int2 rg = a.ranges[r];
         const int32_t qb = uniform(rg.x), qe = uniform(rg.y);
-        if (qb >= c.end) break;  // later cells hold later positions only
+        // a row starting past the chunk's cell holds later positions only;
+        // rows are not always ascending (nearest first), so skip, not stop
+        if (qb >= c.end) continue;
         for (int32_t q = qb; q < qe; ++q) {
             if (!core[q]) continue;
             const int32_t qp = slot_pos(a.pos_of_slot, q);

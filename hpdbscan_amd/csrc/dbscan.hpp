@@ -18,7 +18,7 @@ struct DbscanArgs {
     int32_t nitems = 0;
     const int32_t* ustart = nullptr; // first row of each occupied cell, [m+1]
     const uint32_t* roff = nullptr;  // CSR offsets into ranges, [m+1]
-    const int2* ranges = nullptr;    // [first, end) rows of neighbour rows, ascending
+    const int2* ranges = nullptr;    // [first, end) rows of neighbour rows (ascending, or nearest first)
     const int2* rcells = nullptr;    // [first, end) occupied cells of the same rows
     const int32_t* rcenter = nullptr; // index into ranges of each cell's own row
     int32_t m = 0;                   // occupied cells

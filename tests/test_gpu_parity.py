@@ -110,8 +110,8 @@ def test_integer_dtypes_vs_oracle(ctx, dtype):
 
 def test_dropin_python_float64(ctx):
     """HPDBSCAN(eps, minPts).cluster(ndarray) runs in float64 like the SWIG
-    module and returns a tuple."""
-    import hpdbscan_amd as h
+    module and returns a tuple; `import hpdbscan` is that module."""
+    import hpdbscan as h
     x32 = blobs(5000, 3, 12, 0.25, 6.0, 3, np.float32)
     say("dropin float64 via HPDBSCAN.cluster")
     res = h.HPDBSCAN(0.3, 5).cluster(x32)
@@ -360,6 +360,65 @@ def test_one_thread_rule_point_union_and_integers(ctx1):
     np.testing.assert_array_equal(ctx1.cluster(xi, 2.5, 5), oracle.cluster(xi, 2.5, 5))
 
 
+def lattice_blobs(n, d, k_per_dim, spacing, sigma, noise_frac, seed):
+    """Tight blobs on a k^d lattice with uniform noise between them: few
+    occupied cells with many rows each, and border rows between blobs that
+    lie within eps of cores of two clusters."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(k_per_dim)] * d, indexing="ij"), -1).reshape(-1, d)
+    g = g * spacing
+    nb = int(n * (1 - noise_frac))
+    x = np.concatenate([g[rng.integers(0, len(g), nb)] + rng.normal(0, sigma, (nb, d)),
+                        rng.uniform(-spacing / 2, spacing * (k_per_dim - 0.5), (n - nb, d))])
+    return x[rng.permutation(n)].astype(np.float32)
+
+
+# (d, n, blobs per dim, spacing, sigma, noise, eps, minPts): every case has
+# m * 3^(d-1) <= n (neighbour rows kept from the count pass, nearest first)
+# and border rows where the one-thread choice differs from the smallest
+# cluster (94 rows in 3-D, 9 in 4-D, counted on the CPU)
+DENSE_ONE_THREAD = [(3, 60000, 3, 0.38, 0.03, 0.1, 0.1, 40),
+                    (4, 200000, 3, 0.4, 0.03, 0.05, 0.12, 40)]
+
+
+@pytest.mark.parametrize("case", DENSE_ONE_THREAD, ids=lambda c: f"d{c[0]}")
+def test_one_thread_rule_dense_nearest_first(ctx, ctx1, case):
+    """The border relaxation reads neighbour rows stored nearest first (not
+    ascending); every label equals the reference's one-thread label."""
+    d, n, k, sp, sig, nf, eps, mp = case
+    x = lattice_blobs(n, d, k, sp, sig, nf, 7)
+    say(f"one-thread rule, dense cells d={d} n={n}")
+    full = oracle.cluster(x, eps, mp, full=True)
+    got = ctx1.cluster(x, eps, mp)
+    st = ctx1.stats()
+    assert st["occupied_cells"] * 3 ** (d - 1) <= n
+    assert st["ambiguous_borders"] == full["stats"]["ambiguous_borders"] > 0
+    np.testing.assert_array_equal(got, full["labels"])
+    # the default rule chooses differently on some of these rows
+    assert int(np.count_nonzero(ctx.cluster(x, eps, mp) != got)) > 0
+
+
+def test_loose_boxes_dense_cells_3d(ctx, ctx1):
+    """Loose boxes (x0 near 6e6, float spacing 0.5 > eps) in 3-D with dense
+    cells: the slot-level hook and union read neighbour rows stored nearest
+    first; every label equals the oracle's under both border rules."""
+    rng = np.random.default_rng(11)
+    n = 40000
+    x = np.stack([np.float32(6.0e6) + rng.integers(0, 200, n).astype(np.float32) * np.float32(0.5),
+                  rng.uniform(0, 1.0, n).astype(np.float32),
+                  rng.uniform(0, 1.0, n).astype(np.float32)], axis=1).astype(np.float32)
+    x = np.concatenate([x, np.zeros((1, 3), np.float32)])
+    eps, mp = 0.3, 60  # 186 clusters, 113 border rows next to two of them
+    say(f"loose boxes, dense 3-D cells n={len(x)}")
+    want = oracle.cluster(x, eps, mp)
+    got = ctx.cluster(x, eps, mp)
+    st = ctx.stats()
+    assert st["box_union"] == 0
+    assert st["occupied_cells"] * 9 <= len(x)
+    assert_parity(x, eps, mp, got, want)
+    np.testing.assert_array_equal(ctx1.cluster(x, eps, mp), want)
+
+
 def test_one_thread_rule_batch(ctx1):
     from hpdbscan_amd import _lib
     clouds = [blobs(1500, 2, 40, 0.35, 6.0, s, np.float32, noise=0.3) for s in range(4)]
@@ -390,6 +449,7 @@ def test_server_results_equal_alone_and_in_the_mix(ctx):
     assert st["batches"] < len(reqs)  # requests were batched
     for (x, e, m), (got, info) in zip(reqs, outs):
         np.testing.assert_array_equal(got, ctx.cluster(x, e, m))
+        assert_parity(x, e, m, got, oracle.cluster(x, e, m))
 
 
 def test_server_admission_rejects_oversize(ctx):
