@@ -122,6 +122,7 @@ SIGNATURES = {
                                          _vp, _vp, _vp]),
     "hpd_cluster_batch_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int32,
                                                 ctypes.c_int32, _vp, _vp, _vp, _vp]),
+    "hpd_check_neighbor_rows": (ctypes.c_int, [ctypes.c_uint64]),
     "hpd_device_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "hpd_device_free": (ctypes.c_int, [_vp, _vp]),
     "hpd_memcpy_to_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),

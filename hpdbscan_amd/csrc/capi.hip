@@ -258,6 +258,18 @@ private:
 // --------------------------------------------------------------------------
 // one clustering run
 // --------------------------------------------------------------------------
+// Neighbour rows of one run are indexed with 32-bit offsets (signed in the
+// kernels' loops).
+constexpr uint64_t kMaxNeighborRows = 0x7fffffffull;
+
+static void check_neighbor_rows(uint64_t total) {
+    if (total > kMaxNeighborRows)
+        throw Error(HPD_EUNSUPPORTED,
+                    "the run has " + std::to_string(total) +
+                        " neighbour rows (non-empty rows of 3 adjacent cells, summed over the "
+                        "occupied cells); at most 2^31-1 are supported");
+}
+
 static inline int bits_for(uint64_t max_value) {
     return max_value == 0 ? 0 : 64 - __builtin_clzll(max_value);
 }
@@ -392,7 +404,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // up to 27 rows per cell (D <= 4): the copy puts the nearest rows first
     const bool by_class = stash && D <= 4;
     launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
-                                &dstats[0], stash, stash_rows, rcenter, by_class, s);
+                                &dstats[0], &dstats[4], stash, stash_rows, rcenter, by_class, s);
     launch_chunk_counts(ustart, m, nchunks, s);
     // box mode: dense-cell chunks and sparse-cell rows as separate work
     // lists; their per-cell counts live in the (now unused) run flags
@@ -418,7 +430,12 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     HPD_HIP(hipMemcpyAsync(&counters[1], &nranges[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(&counters[2], &nchunks[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     HPD_HIP(hipMemcpyAsync(hc, counters, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    unsigned long long* hs_rows = reinterpret_cast<unsigned long long*>(ctx.host_scratch) + 12;
+    HPD_HIP(hipMemcpyAsync(hs_rows, &dstats[4], sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
+    // the scan of nranges and every range index are 32-bit: check the
+    // 64-bit total first
+    check_neighbor_rows(*hs_rows);
     const uint32_t total_ranges = hc[1];
     const int32_t ncitems = boxes ? static_cast<int32_t>(hc[11]) : 0;
     const int32_t nitems = static_cast<int32_t>(hc[2]);
@@ -1022,6 +1039,10 @@ int hpd_cluster_batch(hpd_context* ctx, int dtype, const void* data, const int64
         HPD_HIP(hipMemcpyAsync(labels_out, dout, out_bytes, hipMemcpyDeviceToHost, c.stream));
         HPD_HIP(hipStreamSynchronize(c.stream));
     });
+}
+
+int hpd_check_neighbor_rows(uint64_t neighbor_rows) {
+    return guarded([&] { hpd::check_neighbor_rows(neighbor_rows); });
 }
 
 int hpd_device_alloc(hpd_context* ctx, size_t bytes, void** out) {

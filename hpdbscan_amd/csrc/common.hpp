@@ -149,6 +149,34 @@ __device__ __forceinline__ void block_add(unsigned long long v, unsigned long lo
     }
 }
 
+// Two sums at once, to *out0 and *out1 (same rules as block_add).
+template <int kBlockThreads>
+__device__ __forceinline__ void block_add2(unsigned long long v0, unsigned long long v1,
+                                           unsigned long long* out0, unsigned long long* out1) {
+    constexpr int kWaves = kBlockThreads / 64;
+    __shared__ unsigned long long part2[2][kWaves];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 += __shfl_xor(v0, o, 64);
+        v1 += __shfl_xor(v1, o, 64);
+    }
+    if (lane_id() == 0) {
+        part2[0][threadIdx.x >> 6] = v0;
+        part2[1][threadIdx.x >> 6] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t0 = 0, t1 = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            t0 += part2[0][w];
+            t1 += part2[1][w];
+        }
+        if (t0) atomicAdd(out0, t0);
+        if (t1) atomicAdd(out1, t1);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Lock-free union-find over sorted positions.  parent[x] <= x always; a root
 // is hooked only under a smaller root, so every tree's root is its smallest

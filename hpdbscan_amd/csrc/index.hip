@@ -13,8 +13,10 @@
 // the up-to-three cells (i0-1, i0, i0+1) of one "row" occupy one contiguous
 // range of sorted rows.  A cell therefore has at most 3^(D-1) ranges, found
 // by binary search of the row's first and last cell id in the occupied-cell
-// table.  A cell's ranges are stored in ascending row order; the index of
-// the row holding the cell itself is kept separately.
+// table.  A cell's ranges are stored in ascending row order, or (D <= 4 with
+// the count pass's stash) nearest first: the own row, then the rows offset in
+// one, two, three dimensions.  The index of the row holding the cell itself
+// is kept separately.
 #include "common.hpp"
 #include "grid.hpp"
 #include "index.hpp"
@@ -150,6 +152,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              uint32_t* __restrict__ nranges,
                                                              uint32_t* __restrict__ core_chunks,
                                                              unsigned long long* __restrict__ sumcand,
+                                                             unsigned long long* __restrict__ rows_total,
                                                              int2* __restrict__ stash,
                                                              int32_t stash_rows,
                                                              int32_t* __restrict__ center_local,
@@ -157,6 +160,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
+    uint32_t nranges_u = 0;
     if (u < m) {
         const uint64_t key = static_cast<uint64_t>(ucell[u]);
         const int32_t j = job_of_key(jobs, njobs, key);
@@ -186,6 +190,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
             }
         });
         nranges[u] = cnt;
+        nranges_u = cnt;
         if (mine) center_local[u] = center;
         count_u = static_cast<uint64_t>(ustart[u + 1] - ustart[u]);
         // the reference queries a row only with >= minPts candidates
@@ -194,8 +199,11 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
             core_chunks[u] = cand >= jobs[j].min_pts ? static_cast<uint32_t>((count_u + kWave - 1) / kWave)
                                                      : 0u;
     }
-    // Σ over rows of the cell of |candidates| = count_u * cand
-    block_add<256>(static_cast<unsigned long long>(cand * count_u), sumcand);
+    // *sumcand += Σ over rows of the cell of |candidates| = count_u * cand;
+    // *rows_total += the cell's neighbour rows (their 64-bit total is checked
+    // on the host before the 32-bit offsets are used)
+    block_add2<256>(static_cast<unsigned long long>(cand * count_u),
+                    static_cast<unsigned long long>(nranges_u), sumcand, rows_total);
 }
 
 template <typename K, typename T>
@@ -413,11 +421,11 @@ template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           int2* stash, int32_t stash_rows, int32_t* rcenter, bool by_class,
-                           hipStream_t s) {
+                           unsigned long long* rows_total, int2* stash, int32_t stash_rows,
+                           int32_t* rcenter, bool by_class, hipStream_t s) {
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
-                       stash, stash_rows, rcenter, by_class);
+                       rows_total, stash, stash_rows, rcenter, by_class);
     HPD_HIP(hipGetLastError());
 }
 
@@ -468,13 +476,13 @@ HPD_INST_INDEX(uint64_t)
     template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, bool,            \
-                                                     hipStream_t);                              \
+                                                     unsigned long long*, int2*, int32_t,       \
+                                                     int32_t*, bool, hipStream_t);              \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
-                                                     int2*, int32_t, int32_t*, bool,            \
-                                                     hipStream_t);                              \
+                                                     unsigned long long*, int2*, int32_t,       \
+                                                     int32_t*, bool, hipStream_t);              \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \

@@ -19,7 +19,8 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 // job_of_cell[u] = job of cell u; nranges[u] = number of non-empty
 // neighbour rows of cell u; core_chunks[u] (when not null) = 64-row chunks
 // of cell u if it has >= minPts candidates, else 0; adds
-// Σ_u count(u) * |candidates(u)| to *sumcand.  With a stash (m * stash_rows
+// Σ_u count(u) * |candidates(u)| to *sumcand and Σ_u nranges[u] to
+// *rows_total (64-bit, unlike the scan of nranges).  With a stash (m * stash_rows
 // entries, stash_rows >= the most neighbour rows a cell can have) the
 // rows' cell ranges go to stash[u * stash_rows ..] and the own row's index
 // among them to rcenter[u], for launch_neighbor_copy; with by_class (D <= 4)
@@ -29,8 +30,8 @@ template <typename K, typename T>
 void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
-                           int2* stash, int32_t stash_rows, int32_t* rcenter, bool by_class,
-                           hipStream_t s);
+                           unsigned long long* rows_total, int2* stash, int32_t stash_rows,
+                           int32_t* rcenter, bool by_class, hipStream_t s);
 
 // launch_neighbor_fill's output from the count pass's stash (roff scanned);
 // with by_class the rows of each cell go nearest first (own row, then by

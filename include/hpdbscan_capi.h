@@ -224,6 +224,14 @@ int hpd_server_destroy(hpd_server* srv);
 /* Thread-local message of the last failing hpd_server_* call. */
 const char* hpd_server_last_error(void);
 
+/* Size limit of one run's neighbour table (no counterpart in the reference,
+ * which builds candidate lists per thread, spatial_index.h:682-737): a run
+ * whose occupied cells have more than 2^31-1 non-empty neighbour rows in
+ * total (rows of 3 adjacent cells; at most occupied_cells * 3^(d-1)) fails
+ * with HPD_EUNSUPPORTED before any table is sized from the total.  This
+ * applies the same check to a given total: HPD_OK if it fits. */
+int hpd_check_neighbor_rows(uint64_t neighbor_rows);
+
 /* Device-memory helpers for callers of hpd_cluster_device that have no HIP
  * runtime of their own (no counterpart in the reference).  Allocations are
  * on the context's device; copies are ordered on the context's stream and
