@@ -7,6 +7,8 @@
  *   Clusters cluster(T* data, int dim0, int dim1[, int threads])
  *                                                      hpdbscan.h:475-486
  *   Clusters cluster(path, dataset[, threads])         hpdbscan.h:300-304
+ *   Clusters cluster<T>(Dataset&[, int threads])       hpdbscan.h:358-359
+ *                                     (Dataset: include hpdbscan_dataset.h)
  *
  * Same types (Cluster = ptrdiff_t, Clusters = std::vector<Cluster>,
  * constants.h:24-26), same label convention (0 noise, -k core, +k border of
@@ -40,6 +42,11 @@
 
 using Cluster = ptrdiff_t;
 using Clusters = std::vector<Cluster>;
+
+// The reference's in-memory dataset (dataset.h:26-62); defined, with the
+// member template HPDBSCAN::cluster<T>(Dataset&, int), by hpdbscan_dataset.h,
+// which needs hdf5.h as the reference's does.
+struct Dataset;
 
 namespace hpdbscan_amd {
 
@@ -138,6 +145,16 @@ public:
     template <typename T>
     Clusters cluster(T* data, int dim0, int dim1) {
         return cluster(data, dim0, dim1, 0);
+    }
+
+    // cluster<T>(Dataset&, int threads) (hpdbscan.h:358-359): the rows of
+    // dataset.m_p read as T; defined in hpdbscan_dataset.h
+    template <typename T>
+    Clusters cluster(Dataset& dataset, int threads);
+
+    template <typename T>
+    Clusters cluster(Dataset& dataset) {
+        return cluster<T>(dataset, 0);
     }
 };
 

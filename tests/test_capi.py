@@ -148,3 +148,33 @@ def test_batch_argument_errors():
     rc = lib.hpd_cluster_batch(None, 0, x.ctypes.data, off.ctypes.data, 2, 2, eps.ctypes.data,
                                mp.ctypes.data, out.ctypes.data)
     assert rc == _lib.HPD_EINVAL
+
+
+def test_neighbor_row_total_guard():
+    """The neighbour table of a run is indexed with 32-bit offsets; a 64-bit
+    total above 2^31-1 is refused before any table is sized from it."""
+    from hpdbscan_amd import _lib
+    lib = _lib.load()
+    assert lib.hpd_check_neighbor_rows(0) == _lib.HPD_OK
+    assert lib.hpd_check_neighbor_rows(2**31 - 1) == _lib.HPD_OK
+    # 8-D: one million occupied cells with all 3^7 rows non-empty
+    for total in (2**31, 2**32, 2**32 + 5, 1_000_000 * 3**7, 2**64 - 1):
+        assert lib.hpd_check_neighbor_rows(total) == _lib.HPD_EUNSUPPORTED
+        msg = lib.hpd_last_error().decode()
+        assert str(total) in msg and "2^31-1" in msg
+
+
+def test_reference_module_name():
+    """`import hpdbscan` works as with the reference's SWIG module
+    (swig/hpdbscan.i:1, README.md:69-72) and exposes the same class."""
+    import importlib
+    import hpdbscan
+    import hpdbscan_amd.hpdbscan as impl
+    importlib.reload(hpdbscan)
+    assert hpdbscan.HPDBSCAN is impl.HPDBSCAN
+    c = hpdbscan.HPDBSCAN(epsilon=0.3, min_points=4)
+    assert c.min_points == 4
+    with pytest.raises(ValueError, match="epsilon needs to be positive"):
+        hpdbscan.HPDBSCAN(0, 4)
+    with pytest.raises(TypeError, match="2 dimensions"):
+        c.cluster(np.zeros(5))
