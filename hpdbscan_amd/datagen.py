@@ -85,7 +85,9 @@ def skewed_2d(n: int = 100_000_000, seed: int = 3, chunk: int = 10_000_000) -> n
         out[pos:pos + m] = rng.uniform(0.0, 1.0, size=(m, 2))
         pos += m
         done += m
-    rng.shuffle(out, axis=0)
+    # a row of two float32 is one 8-byte word: shuffling the words is the
+    # same permutation as shuffling the rows (same draws), 30x faster
+    rng.shuffle(out.view(np.int64).reshape(-1))
     return out
 
 

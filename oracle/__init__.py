@@ -77,6 +77,11 @@ def _lib():
         lib.hpd_oracle_check.argtypes = [ctypes.c_int, vp, i64, i32, ctypes.c_float, u64, vp,
                                          ctypes.POINTER(u64), ctypes.POINTER(u64)]
         lib.hpd_oracle_check.restype = ctypes.c_int
+        lib.hpd_oracle_check_parallel.argtypes = [ctypes.c_int, vp, i64, i32, ctypes.c_float, u64,
+                                                  vp, ctypes.c_int, ctypes.POINTER(u64),
+                                                  ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                                  ctypes.POINTER(u64)]
+        lib.hpd_oracle_check_parallel.restype = ctypes.c_int
         _oracle = lib
     return _oracle
 
@@ -134,6 +139,28 @@ def check(data, eps: float, min_pts: int, got) -> tuple[int, int]:
     if rc != 0:
         raise ValueError(f"oracle check failed with code {rc}")
     return int(m.value), int(b.value)
+
+
+def check_parallel(data, eps: float, min_pts: int, got, threads: int = 0) -> dict:
+    """Like ``check`` for inputs too large for the one-thread restatement:
+    ``got`` is compared with the closed form of the reference's result
+    (core flags, components of cores, cluster ids, noise, one-cluster
+    borders exact; an ambiguous border must carry a cluster with a core
+    within eps), computed with ``threads`` OpenMP threads (0: all).
+    Returns {"mismatches", "bad_ambiguous", "cores", "ambiguous"}."""
+    a, (code, _) = _prep(data)
+    n, d = a.shape
+    g = np.ascontiguousarray(got, dtype=np.int64)
+    if g.shape != (n,):
+        raise ValueError("labels must have one entry per row")
+    m, b, c, am = (ctypes.c_uint64() for _ in range(4))
+    rc = _lib().hpd_oracle_check_parallel(code, a.ctypes.data, n, d, ctypes.c_float(eps), min_pts,
+                                          g.ctypes.data, int(threads), ctypes.byref(m),
+                                          ctypes.byref(b), ctypes.byref(c), ctypes.byref(am))
+    if rc != 0:
+        raise ValueError(f"oracle parallel check failed with code {rc}")
+    return {"mismatches": int(m.value), "bad_ambiguous": int(b.value), "cores": int(c.value),
+            "ambiguous": int(am.value)}
 
 
 # --------------------------------------------------------------------------

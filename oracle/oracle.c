@@ -50,6 +50,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 #include <xmmintrin.h>
 
 #define HPD_NOT_VISITED INT64_MAX
@@ -805,5 +806,175 @@ int hpd_oracle_check(int dt, const void* data, int64_t n, int32_t d, float eps, 
     free(pos);
     if (mismatched_exact) *mismatched_exact = mism;
     if (bad_ambiguous) *bad_ambiguous = bad;
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* parallel checker of the closed form (large inputs)                        */
+/* ------------------------------------------------------------------------ */
+
+/* Lock-free union-find over sorted positions: a root is always the smallest
+ * position of its tree (the larger root is hooked under the smaller). */
+static int64_t uf_find_par(int64_t* parent, int64_t x) {
+    for (;;) {
+        const int64_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+        if (p == x) return x;
+        const int64_t g = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+        if (g != p) __atomic_compare_exchange_n(&parent[x], &(int64_t){p}, g, 0, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED);
+        x = g;
+    }
+}
+
+static void uf_unite_par(int64_t* parent, int64_t a, int64_t b) {
+    for (;;) {
+        a = uf_find_par(parent, a);
+        b = uf_find_par(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            const int64_t t = a;
+            a = b;
+            b = t;
+        }
+        int64_t expect = a;
+        if (__atomic_compare_exchange_n(&parent[a], &expect, b, 0, __ATOMIC_ACQ_REL,
+                                        __ATOMIC_RELAXED))
+            return;
+    }
+}
+
+/*
+ * Checker of candidate labels `got` (input order) against the closed form of
+ * the reference's result, computed with `threads` OpenMP threads (0: the
+ * default), for inputs too large for the one-thread restatement above:
+ *   core(p)  <=>  |{q in candidates(p) : dist2(p, q) <= EPS2}| >= minPts
+ *   label of a core = -(1 + smallest sorted position of a core of its
+ *                     eps-connected component of cores)
+ *   non-core row: 0 without a core within eps; +k if the cores within eps
+ *                 all lie in cluster k; else (ambiguous) any +k of them.
+ * Counts like hpd_oracle_check: mismatched_exact (core, noise, core labels,
+ * one-cluster borders) and bad_ambiguous.  Optional: number of cores and of
+ * ambiguous rows.  The candidates, the distance test and the sorted order
+ * are those of the restatement above (build_spatial, neighbor_ranges,
+ * within); the core count stops at minPts, the union tests each core pair
+ * once.  Returns 0, or 1/2 like hpd_oracle_cluster.
+ */
+int hpd_oracle_check_parallel(int dt, const void* data, int64_t n, int32_t d, float eps,
+                              uint64_t min_pts, const int64_t* got, int threads,
+                              uint64_t* mismatched_exact, uint64_t* bad_ambiguous,
+                              uint64_t* cores_out, uint64_t* ambiguous_out) {
+    if (!(eps > 0.0f)) return 1;
+    if (d < 1 || d > 10 || n < 0) return 1;
+    const size_t esz = dtype_size(dt);
+    if (esz == 0) return 1;
+    if (threads > 0) omp_set_num_threads(threads);
+    const unsigned int saved_csr = _mm_getcsr();
+    _mm_setcsr(saved_csr | 0x8040);
+    spatial s;
+    memset(&s, 0, sizeof(s));
+    s.n = n;
+    s.d = d;
+    s.dt = dt;
+    s.esz = esz;
+    s.eps = eps;
+    int rc = 0;
+    uint8_t* core = NULL;
+    int64_t* parent = NULL;
+    uint64_t mism = 0, bad = 0, ncore = 0, namb = 0;
+    if (build_spatial(&s, data) != 0) {
+        rc = 2;
+        goto done;
+    }
+    core = (uint8_t*)calloc((size_t)(n + 1), 1);
+    parent = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    if (!core || !parent) {
+        rc = 2;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; ++i) parent[i] = i;
+    const eps2_t e2 = make_eps2(dt, eps);
+    int64_t n3 = 1;
+    for (int j = 0; j < d; ++j) n3 *= 3;
+    const int64_t ncells = s.n_cells;
+    int oom = 0;
+
+    /* pass 1: core flags; pass 2: union of core pairs; pass 3: labels */
+    for (int pass = 1; pass <= 3 && !oom; ++pass) {
+#pragma omp parallel reduction(+ : mism, bad, ncore, namb) reduction(| : oom)
+        {
+            const unsigned int thread_csr = _mm_getcsr();
+            _mm_setcsr(thread_csr | 0x8040);
+            uint64_t* sc = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n3);
+            int64_t* rs = (int64_t*)malloc(sizeof(int64_t) * (size_t)n3);
+            int64_t* rcn = (int64_t*)malloc(sizeof(int64_t) * (size_t)n3);
+            if (!sc || !rs || !rcn) oom = 1;
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t u = 0; u < ncells; ++u) {
+                if (!sc || !rs || !rcn) continue;
+                const int64_t nr = neighbor_ranges(&s, s.ucell[u], sc, rs, rcn);
+                const int64_t p0 = s.ustart[u], p1 = p0 + s.ucount[u];
+                for (int64_t p = p0; p < p1; ++p) {
+                    if (pass == 1) {
+                        uint64_t cnt = 0;
+                        for (int64_t r = 0; r < nr && cnt < min_pts; ++r)
+                            for (int64_t q = rs[r]; q < rs[r] + rcn[r] && cnt < min_pts; ++q)
+                                cnt += within(&s, p, q, &e2) ? 1u : 0u;
+                        core[p] = cnt >= min_pts;
+                    } else if (pass == 2) {
+                        if (!core[p]) continue;
+                        for (int64_t r = 0; r < nr; ++r)
+                            for (int64_t q = rs[r]; q < rs[r] + rcn[r] && q < p; ++q) {
+                                if (!core[q]) continue;
+                                if (uf_find_par(parent, q) == uf_find_par(parent, p)) continue;
+                                if (within(&s, p, q, &e2)) uf_unite_par(parent, p, q);
+                            }
+                    } else {
+                        const int64_t g = got[s.orig_of_sorted[p]];
+                        if (core[p]) {
+                            ++ncore;
+                            if (g != -(uf_find_par(parent, p) + 1)) ++mism;
+                            continue;
+                        }
+                        int64_t k0 = -1;   /* first adjacent cluster (root) */
+                        int many = 0;      /* two or more adjacent clusters */
+                        int got_adj = 0;   /* g = +k for an adjacent cluster k */
+                        for (int64_t r = 0; r < nr; ++r)
+                            for (int64_t q = rs[r]; q < rs[r] + rcn[r]; ++q) {
+                                if (!core[q] || !within(&s, p, q, &e2)) continue;
+                                const int64_t k = uf_find_par(parent, q);
+                                if (k0 < 0) k0 = k;
+                                else if (k != k0) many = 1;
+                                if (g == k + 1) got_adj = 1;
+                            }
+                        if (!many) {
+                            if (g != (k0 < 0 ? 0 : k0 + 1)) ++mism;
+                        } else {
+                            ++namb;
+                            if (!got_adj) ++bad;
+                        }
+                    }
+                }
+            }
+            free(sc);
+            free(rs);
+            free(rcn);
+            _mm_setcsr(thread_csr);
+        }
+        if (pass == 2) {
+            /* flatten: every core points at its root */
+            for (int64_t i = 0; i < n; ++i)
+                if (core[i]) parent[i] = uf_find_par(parent, i);
+        }
+    }
+    if (oom) rc = 2;
+done:
+    free(core);
+    free(parent);
+    free_spatial(&s);
+    _mm_setcsr(saved_csr);
+    if (mismatched_exact) *mismatched_exact = mism;
+    if (bad_ambiguous) *bad_ambiguous = bad;
+    if (cores_out) *cores_out = ncore;
+    if (ambiguous_out) *ambiguous_out = namb;
     return rc;
 }
