@@ -3,8 +3,7 @@
     python -m hpdbscan_amd.build [--jobs N] [--verbose]
 
 Objects go to hpdbscan_amd/_build/, the library next to this file so that it
-travels with the source tree, and so does `copy_probe` (a plain copy kernel
-of known size that calibrates the HBM traffic counters).  Where libhdf5 1.10
+travels with the source tree.  Where libhdf5 1.10
 is installed (HDF5_PREFIX, default /opt/conda) the HDF5 file layer
 libhpdbscan_amd_h5.so and the command-line tool bin/hpdbscan are built too;
 libhdf5 is linked through _h5lib/, a directory of links to that one library
@@ -31,7 +30,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhpdbscan_amd.so")
-COPY_PROBE = os.path.join(HERE, "copy_probe")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 H5_LIB = os.path.join(HERE, "libhpdbscan_amd_h5.so")
 H5_LINKS = os.path.join(HERE, "_h5lib")
@@ -98,7 +96,6 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
-    build_copy_probe(verbose)
     if hdf5_available():
         build_h5(verbose)
         build_cli(verbose)
@@ -157,19 +154,6 @@ def build_cli(verbose: bool = False) -> str:
          "g++ on cli.cpp", verbose)
     os.replace(tmp, CLI)
     return CLI
-
-
-def build_copy_probe(verbose: bool = False) -> str:
-    """The copy kernel of known size that calibrates the HBM traffic counters."""
-    tmp = COPY_PROBE + ".tmp"
-    cmd = [hipcc(), *compile_flags(), "-x", "hip", os.path.join(CSRC, "copy_probe.cpp"), "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on copy_probe.cpp:\n{r.stderr}")
-    os.replace(tmp, COPY_PROBE)
-    return COPY_PROBE
 
 
 def main() -> None:
