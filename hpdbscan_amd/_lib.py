@@ -118,6 +118,9 @@ SIGNATURES = {
                                    ctypes.c_float, ctypes.c_uint64, _vp]),
     "hpd_cluster_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.c_float, ctypes.c_uint64, _vp, _vp]),
+    "hpd_cluster_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64,
+                                                  ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
+                                                  _vp, _vp]),
     "hpd_cluster_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int32, ctypes.c_int32,
                                          _vp, _vp, _vp]),
     "hpd_cluster_batch_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int32,
@@ -257,6 +260,16 @@ class Context:
         check(load().hpd_cluster_device(self._h, code, _vp(data_ptr), n, d, ctypes.c_float(eps),
                                         int(min_points), _vp(labels_ptr),
                                         _vp(stream) if stream else None))
+
+    def cluster_device_noalloc(self, data_ptr: int, n: int, d: int, dtype, eps: float,
+                               min_points: int, labels_ptr: int, stream: int = 0) -> None:
+        """cluster_device with every workspace buffer in the library's static
+        device scratch (no device allocation); inputs too large for it raise
+        NotImplementedError."""
+        code = DTYPE_CODES[np.dtype(dtype)]
+        check(load().hpd_cluster_device_noalloc(self._h, code, _vp(data_ptr), n, d,
+                                                ctypes.c_float(eps), int(min_points),
+                                                _vp(labels_ptr), _vp(stream) if stream else None))
 
 
 def _batch_args(offsets, eps, min_points):

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -44,12 +45,16 @@ void check_hip(hipError_t e, const char* what) {
 // arena
 // --------------------------------------------------------------------------
 Arena::~Arena() {
-    if (base_) (void)hipFree(base_);
+    if (base_ && owned_) (void)hipFree(base_);
 }
 
 void Arena::reserve(size_t bytes) {
     used_ = 0;
     if (bytes <= cap_) return;
+    if (!owned_)
+        throw Error(HPD_EUNSUPPORTED, "the run needs " + std::to_string(bytes) +
+                                          " bytes of device workspace here; " +
+                                          std::to_string(cap_) + " are available");
     if (base_) {
         HPD_HIP(hipFree(base_));
         base_ = nullptr;
@@ -161,12 +166,12 @@ static Geometry make_geometry(const T* mn, const T* mx, int d, float eps, float*
     g.overflow = overflow;
     // dimension order: the reference's own call, std::sort ascending by cell
     // count (spatial_index.h:135-139)
-    std::vector<size_t> sw(static_cast<size_t>(d));
-    std::iota(sw.begin(), sw.end(), 0);
-    std::sort(sw.begin(), sw.end(), [&](size_t a, size_t b) { return g.cells[a] < g.cells[b]; });
+    size_t sw[kMaxDims];
+    std::iota(sw, sw + d, size_t(0));
+    std::sort(sw, sw + d, [&](size_t a, size_t b) { return g.cells[a] < g.cells[b]; });
     uint64_t acc = 1;
     for (int k = 0; k < d; ++k) {
-        g.swapped[k] = static_cast<int>(sw[static_cast<size_t>(k)]);
+        g.swapped[k] = static_cast<int>(sw[k]);
         g.stride[k] = acc;
         g.ncell[k] = g.cells[g.swapped[k]];
         acc *= g.ncell[k];
@@ -175,6 +180,29 @@ static Geometry make_geometry(const T* mn, const T* mx, int d, float eps, float*
     g.key_bits = maxkey == 0 ? 0 : 64 - __builtin_clzll(maxkey);
     return g;
 }
+
+// Host array of n elements: inline storage up to N (a single job needs no
+// heap allocation), the heap beyond.
+template <typename U, size_t N>
+class HostArray {
+public:
+    explicit HostArray(size_t n) : n_(n) {
+        if (n > N) heap_.resize(n);
+        p_ = n > N ? heap_.data() : inl_;
+        for (size_t i = 0; i < n; ++i) p_[i] = U{};
+    }
+    HostArray(const HostArray&) = delete;
+    HostArray& operator=(const HostArray&) = delete;
+    U* data() { return p_; }
+    U& operator[](size_t i) { return p_[i]; }
+    size_t size() const { return n_; }
+
+private:
+    size_t n_;
+    U inl_[N];
+    std::vector<U> heap_;
+    U* p_;
+};
 
 // --------------------------------------------------------------------------
 // context
@@ -185,6 +213,10 @@ struct Context {
     Arena main;
     Arena aux;
     Arena io;  // input rows and labels of the host-memory entry point
+    // the arenas a run takes its workspace from: main / aux, or the static
+    // scratch (hpd_cluster_device_noalloc)
+    Arena* main_use = &main;
+    Arena* aux_use = &aux;
     int border_rule = HPD_BORDER_MIN_CLUSTER;
     // box mode: cells with at most this many rows are handled one thread
     // per row in the border pass (and in the core count if sparse_core),
@@ -304,7 +336,8 @@ template <typename T, int D, typename K>
 static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>* jobs,
                       int32_t njobs, int key_bits, bool boxes, int64_t* labels, hipStream_t s,
                       PhaseTimer& tm) {
-    Arena& A = ctx.main;
+    Arena& A = *ctx.main_use;
+    Arena& X = *ctx.aux_use;
     const size_t nn = static_cast<size_t>(n);
     const int64_t ld = (n + 15) & ~int64_t(15);  // rows are read in aligned batches of <= 16
     const size_t L = static_cast<size_t>(ld);
@@ -450,21 +483,21 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         nbitems = static_cast<int32_t>(hc[4]);
     }
 
-    ctx.aux.reserve(2 * align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
+    X.reserve(2 * align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nditems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(ncitems) * sizeof(int2), 256) + 2048);
-    int2* ranges = ctx.aux.take_n<int2>(total_ranges);
-    int2* rcells = ctx.aux.take_n<int2>(total_ranges);
-    int2* items = ctx.aux.take_n<int2>(static_cast<size_t>(nitems));
-    int2* bitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
-    int2* ditems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nditems)) : nullptr;
-    int2* sitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
-    int2* uitems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
-    int2* citems = boxes ? ctx.aux.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
+    int2* ranges = X.take_n<int2>(total_ranges);
+    int2* rcells = X.take_n<int2>(total_ranges);
+    int2* items = X.take_n<int2>(static_cast<size_t>(nitems));
+    int2* bitems = boxes ? X.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
+    int2* ditems = boxes ? X.take_n<int2>(static_cast<size_t>(nditems)) : nullptr;
+    int2* sitems = boxes ? X.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
+    int2* uitems = boxes ? X.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
+    int2* citems = boxes ? X.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
     if (stash)
         launch_neighbor_copy(stash, stash_rows, ustart, m, nranges, ranges, rcells, rcenter,
                              by_class, s);
@@ -659,13 +692,13 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
                       hipStream_t s) {
     PhaseTimer tm(ctx, s);
     tm.mark(M_START);
-    Arena& A = ctx.main;
+    Arena& A = *ctx.main_use;
     A.reserve(workspace_bytes<T, D>(n, njobs, ctx.border_rule == HPD_BORDER_ONE_THREAD));
 
     // job table, first pass: row ranges and min/max blocks.  The host
     // vectors outlive every copy from them: the run ends with a stream sync.
-    std::vector<JobParams<T>> hj(static_cast<size_t>(njobs));
-    std::vector<int32_t> bfirst(static_cast<size_t>(njobs) + 1, 0);
+    HostArray<JobParams<T>, 1> hj(static_cast<size_t>(njobs));
+    HostArray<int32_t, 2> bfirst(static_cast<size_t>(njobs) + 1);
     const int rpb = minmax_rows_per_block();
     int32_t nblocks = 0;
     for (int32_t j = 0; j < njobs; ++j) {
@@ -688,7 +721,7 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
 
     // min / max per job (spatial_index.h:99-122)
     launch_minmax<T, D>(pts, jobs_dev, njobs, bfirst_dev, nblocks, part, mm, nonfinite, s);
-    std::vector<T> hmm(static_cast<size_t>(njobs) * 2 * D);
+    HostArray<T, 2 * D> hmm(static_cast<size_t>(njobs) * 2 * D);
     int hbad = 0;
     HPD_HIP(hipMemcpyAsync(hmm.data(), mm, sizeof(T) * njobs * 2 * D, hipMemcpyDeviceToHost, s));
     HPD_HIP(hipMemcpyAsync(ctx.host_scratch, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -811,6 +844,97 @@ static void run_any(Context& ctx, int dt, const void* pts, int64_t n, int d,
 #undef HPD_RUN_CASE
 }
 
+// workspace_bytes of a plain call (one job) of any dtype and dimension
+template <typename T>
+static size_t workspace_typed(int64_t n, int d, bool one_thread) {
+    switch (d) {
+        case 1: return workspace_bytes<T, 1>(n, 1, one_thread);
+        case 2: return workspace_bytes<T, 2>(n, 1, one_thread);
+        case 3: return workspace_bytes<T, 3>(n, 1, one_thread);
+        case 4: return workspace_bytes<T, 4>(n, 1, one_thread);
+        case 5: return workspace_bytes<T, 5>(n, 1, one_thread);
+        case 6: return workspace_bytes<T, 6>(n, 1, one_thread);
+        case 7: return workspace_bytes<T, 7>(n, 1, one_thread);
+        case 8: return workspace_bytes<T, 8>(n, 1, one_thread);
+        default:
+            throw Error(HPD_EUNSUPPORTED, "dimensions above " + std::to_string(kMaxDims) +
+                                              " are not supported by this build");
+    }
+}
+
+static size_t workspace_any(int dt, int64_t n, int d, bool one_thread) {
+    switch (dt) {
+        case HPD_DTYPE_F32: return workspace_typed<float>(n, d, one_thread);
+        case HPD_DTYPE_F64: return workspace_typed<double>(n, d, one_thread);
+        case HPD_DTYPE_I8: return workspace_typed<int8_t>(n, d, one_thread);
+        case HPD_DTYPE_I16: return workspace_typed<int16_t>(n, d, one_thread);
+        case HPD_DTYPE_I32: return workspace_typed<int32_t>(n, d, one_thread);
+        case HPD_DTYPE_I64: return workspace_typed<int64_t>(n, d, one_thread);
+        case HPD_DTYPE_U8: return workspace_typed<uint8_t>(n, d, one_thread);
+        case HPD_DTYPE_U16: return workspace_typed<uint16_t>(n, d, one_thread);
+        case HPD_DTYPE_U32: return workspace_typed<uint32_t>(n, d, one_thread);
+        case HPD_DTYPE_U64: return workspace_typed<uint64_t>(n, d, one_thread);
+        default: throw Error(HPD_EINVAL, "unknown dtype code " + std::to_string(dt));
+    }
+}
+
+// --------------------------------------------------------------------------
+// static scratch of hpd_cluster_device_noalloc
+// --------------------------------------------------------------------------
+// Every workspace buffer of a run of hpd_cluster_device_noalloc lives in
+// this array (one copy per device, held until the process ends).  Host code
+// never passes its name to a kernel: the name is a host-side stand-in; the
+// device address comes from hipGetSymbolAddress, looked up once per device
+// when a context is created on it (with that device current).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static void init_scratch_for_current_device(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw Error(HPD_EUNSUPPORTED, "device index above " + std::to_string(kMaxDevices - 1));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    HPD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(HPD_EDEVICE, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[device] = ptr;
+}
+
+static void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// Points a context's runs at fixed arenas and turns its timing off (event
+// creation) for the duration of one call.
+class FixedWorkspace {
+public:
+    FixedWorkspace(Context& c, void* base, size_t main_bytes, size_t total_bytes)
+        : c_(c), timing_(c.timing) {
+        main_.attach(base, main_bytes);
+        aux_.attach(static_cast<char*>(base) + main_bytes, total_bytes - main_bytes);
+        c.main_use = &main_;
+        c.aux_use = &aux_;
+        c.timing = false;
+    }
+    ~FixedWorkspace() {
+        c_.main_use = &c_.main;
+        c_.aux_use = &c_.aux;
+        c_.timing = timing_;
+    }
+    FixedWorkspace(const FixedWorkspace&) = delete;
+    FixedWorkspace& operator=(const FixedWorkspace&) = delete;
+
+private:
+    Context& c_;
+    bool timing_;
+    Arena main_, aux_;
+};
+
 static void validate(const Context* ctx, int dt, const void* data, int64_t n, int32_t d, float eps,
                      const void* labels) {
     if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
@@ -875,6 +999,7 @@ int hpd_context_create(int device, hpd_context** out) {
         try {
             c->impl.device = device;
             hpd::DeviceGuard g(device);
+            hpd::init_scratch_for_current_device(device);
             HPD_HIP(hipStreamCreateWithFlags(&c->impl.stream, hipStreamNonBlocking));
             void* hs = nullptr;
             HPD_HIP(hipHostMalloc(&hs, 64 * sizeof(uint64_t), hipHostMallocDefault));
@@ -953,6 +1078,37 @@ int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n,
         c.stats.d = d;
         if (n == 0) return;
         hpd::DeviceGuard g(c.device);
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
+        const int64_t offsets[2] = {0, n};
+        hpd::run_any(c, dtype, data, n, d, offsets, 1, &eps, &min_pts, labels_out, s);
+    });
+}
+
+int hpd_cluster_device_noalloc(hpd_context* ctx, int dtype, const void* data, int64_t n,
+                               int32_t d, float eps, uint64_t min_pts, int64_t* labels_out,
+                               void* stream) {
+    return guarded([&] {
+        hpd::validate(ctx ? &ctx->impl : nullptr, dtype, data, n, d, eps, labels_out);
+        Context& c = ctx->impl;
+        c.stats = hpd_run_stats{};
+        c.stats.n = n;
+        c.stats.d = d;
+        if (n == 0) return;
+        // the workspace of the main arena, checked before anything runs; the
+        // neighbour tables and work lists take the rest and are checked when
+        // their sizes are known (before any label is written)
+        const size_t need = hpd::workspace_any(dtype, n, d, c.border_rule == HPD_BORDER_ONE_THREAD);
+        if (need > hpd::kScratchBytes)
+            throw Error(HPD_EUNSUPPORTED,
+                        "hpd_cluster_device_noalloc: " + std::to_string(n) + " rows of " +
+                            std::to_string(d) + " dimensions need " + std::to_string(need) +
+                            " bytes of workspace; its static scratch holds " +
+                            std::to_string(hpd::kScratchBytes) +
+                            " bytes (use hpd_cluster_device for larger inputs)");
+        hpd::DeviceGuard g(c.device);
+        void* base = hpd::scratch_of(c.device);
+        if (!base) throw Error(HPD_EDEVICE, "no scratch address for this device");
+        hpd::FixedWorkspace ws(c, base, hpd::align_up(need, 256), hpd::kScratchBytes);
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
         const int64_t offsets[2] = {0, n};
         hpd::run_any(c, dtype, data, n, d, offsets, 1, &eps, &min_pts, labels_out, s);

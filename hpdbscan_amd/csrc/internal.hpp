@@ -25,6 +25,8 @@ void check_hip(hipError_t e, const char* what);
 
 // Grow-only device arena.  Allocations are 256-byte aligned bump pointers
 // into one hipMalloc'd block; reset() recycles the block for the next call.
+// An arena attached to memory it does not own never allocates: reserve()
+// beyond its capacity throws HPD_EUNSUPPORTED.
 class Arena {
 public:
     Arena() = default;
@@ -32,6 +34,13 @@ public:
     Arena(const Arena&) = delete;
     Arena& operator=(const Arena&) = delete;
 
+    // Use [base, base + cap) (256-byte aligned) instead of allocating.
+    void attach(void* base, size_t cap) {
+        base_ = static_cast<char*>(base);
+        cap_ = cap;
+        used_ = 0;
+        owned_ = false;
+    }
     // Make sure at least `bytes` are available in one block (drops contents).
     void reserve(size_t bytes);
     void reset() { used_ = 0; }
@@ -47,6 +56,7 @@ private:
     char* base_ = nullptr;
     size_t cap_ = 0;
     size_t used_ = 0;
+    bool owned_ = true;
 };
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }

@@ -147,6 +147,22 @@ int hpd_cluster(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_
 int hpd_cluster_device(hpd_context* ctx, int dtype, const void* data, int64_t n, int32_t d,
                        float eps, uint64_t min_pts, int64_t* labels_out, void* stream);
 
+/* hpd_cluster_device without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, same labels.
+ * Every workspace buffer comes from a static device array of 128 MiB that
+ * the library holds on each device it runs on until the process ends; the
+ * call allocates neither device nor host memory.  Inputs whose workspace
+ * does not fit (about 400k rows of 3-D float32 data; the bound depends on n,
+ * d, the element type and the border rule) fail with HPD_EUNSUPPORTED before
+ * any work, or, when the neighbour tables of the data do not fit the rest,
+ * before any label is written.  All calls on one device share the array:
+ * calls that overlap in time on one device (from several threads, contexts
+ * or streams) race and must be serialised by the caller. */
+int hpd_cluster_device_noalloc(hpd_context* ctx, int dtype, const void* data, int64_t n,
+                               int32_t d, float eps, uint64_t min_pts, int64_t* labels_out,
+                               void* stream);
+
 /*
  * Batched clustering: njobs independent point clouds of the same dtype and
  * dimension, concatenated row-wise; job j is rows [offsets[j], offsets[j+1])
