@@ -121,11 +121,12 @@ def kernel_roofline(st: dict, med: dict, n: int, d: int, s: int, workload: str) 
 
 
 def cpu_threads() -> int:
+    """Every host CPU this process may run on."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, n)
 
 
 def cpu_model() -> str:
@@ -136,6 +137,17 @@ def cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def host_info() -> dict:
+    """nproc, the CPUs this process may use, the CPU model and SMT state."""
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": cpu_threads(), "cpu": cpu_model(),
+            "smt_active": smt}
 
 
 def slab_sample(x: np.ndarray, target: int) -> np.ndarray:
@@ -275,6 +287,164 @@ def serving_leg(a) -> dict:
     }
 
 
+def device_steps(ctx, x, eps, mp, steps, warmup, timing=True):
+    """Clusterings of x with rows and labels resident in device memory
+    (hpd_cluster_device), bracketed by stream synchronisation.  Returns
+    (Mpoints/s, ms per step, per-step stats, labels)."""
+    n, d = x.shape
+    xd = _lib.DeviceBuffer.from_array(ctx, x)
+    ld = _lib.DeviceBuffer(ctx, 8 * n)
+    try:
+        ctx.set_timing(timing)
+        for _ in range(warmup):
+            ctx.cluster_device(xd.ptr, n, d, x.dtype, eps, mp, ld.ptr)
+        ctx.synchronize()
+        phases = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.cluster_device(xd.ptr, n, d, x.dtype, eps, mp, ld.ptr)
+            phases.append(ctx.stats())
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        ctx.set_timing(False)
+        labels = ld.to_array(np.int64, n)
+    finally:
+        xd.close()
+        ld.close()
+    return n * steps / elapsed / 1e6, 1e3 * elapsed / steps, phases, labels
+
+
+def host_to_host(ctx, x, eps, mp, reps) -> dict:
+    """The reference's boundary: host rows in, host labels out (hpd_cluster,
+    pageable numpy buffers), with the two copies timed on their own."""
+    n, d = x.shape
+    ctx.cluster(x, eps, mp)  # warm-up (workspace)
+    t_all, t_h2d, t_d2h = [], [], []
+    xb = _lib.DeviceBuffer(ctx, x.nbytes)
+    lb = _lib.DeviceBuffer(ctx, 8 * n)
+    try:
+        for _ in range(reps):
+            t = time.perf_counter()
+            ctx.cluster(x, eps, mp)
+            t_all.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            xb.copy_from(x)
+            t_h2d.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            lb.to_array(np.int64, n)
+            t_d2h.append(time.perf_counter() - t)
+    finally:
+        xb.close()
+        lb.close()
+    med = float(np.median(t_all))
+    return {"value": round(n / med / 1e6, 2), "unit": "Mpoints/s", "ms_per_call": round(1e3 * med, 3),
+            "h2d_ms": round(1e3 * float(np.median(t_h2d)), 3),
+            "d2h_ms": round(1e3 * float(np.median(t_d2h)), 3),
+            "h2d_bytes": int(x.nbytes), "d2h_bytes": 8 * n,
+            "note": "hpd_cluster on pageable host arrays (rows copied in, labels copied out, "
+                    "inside the call); h2d/d2h: the same copies timed alone"}
+
+
+def reference_leg(sample, eps, mp, threads, reps, what) -> dict:
+    res, labels, err = run_reference(sample, eps, mp, threads, reps)
+    if res is None:
+        return {"value": None, "error": err, "sample": what}
+    return {"value": round(len(sample) / res["median_s"] / 1e6, 4), "unit": "Mpoints/s",
+            "threads": threads, "rows": int(len(sample)), "times_s": res["times_s"],
+            "sample": what, "labels": labels}
+
+
+def parity_fields(gpu_labels, ref_labels) -> dict:
+    return {"core_flags_equal": bool(np.array_equal(gpu_labels < 0, ref_labels < 0)),
+            "noise_equal": bool(np.array_equal(gpu_labels == 0, ref_labels == 0)),
+            "core_ids_equal": bool(np.array_equal(gpu_labels[gpu_labels < 0],
+                                                  ref_labels[gpu_labels < 0])),
+            "label_mismatches": int((gpu_labels != ref_labels).sum())}
+
+
+def p1_leg(a) -> dict:
+    """The 100k x 3-D workload: GPU (device-resident and host-to-host) and
+    the reference on all host threads and on one, full size."""
+    x, eps, mp = datagen.make("blobs_100k")
+    ctx = _lib.Context(0)
+    try:
+        v, ms, _, labels = device_steps(ctx, x, eps, mp, 50, 5, timing=False)
+        hh = host_to_host(ctx, x, eps, mp, 10)
+    finally:
+        ctx.close()
+    out = {"workload": "blobs_100k", "n": len(x), "d": 3, "eps": eps, "min_points": mp,
+           "gpu": {"value": round(v, 2), "unit": "Mpoints/s", "ms_per_step": round(ms, 4),
+                   "input": "device-resident"},
+           "gpu_host_to_host": hh}
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or cpu_threads()
+        print(f"[bench] blobs_100k: reference, {threads} threads and 1 thread", file=sys.stderr,
+              flush=True)
+        ref = reference_leg(x, eps, mp, threads, 3, "the full 100k workload")
+        ref1 = reference_leg(x, eps, mp, 1, 1, "the full 100k workload")
+        out["cpu"] = {k: v for k, v in ref.items() if k != "labels"}
+        out["cpu_one_thread"] = {k: v for k, v in ref1.items() if k != "labels"}
+        if ref1.get("labels") is not None:
+            # at one thread the reference's labels are fixed: all equal
+            out["parity_one_thread_ref"] = parity_fields(labels, ref1["labels"])
+        if ref.get("value"):
+            out["speedup_vs_cpu"] = round(v / ref["value"], 1)
+    return out
+
+
+def p3_leg(a) -> dict:
+    """The 100M x 2-D skewed workload on the GPU; the reference on a
+    density-preserving slab, its time scaled to the full workload by the
+    reference's own work measure (candidate distance tests)."""
+    t0 = time.time()
+    x, eps, mp = datagen.make("skewed_2d")
+    n = len(x)
+    print(f"[bench] skewed_2d: n={n} (generated in {time.time() - t0:.1f}s)", file=sys.stderr,
+          flush=True)
+    ctx = _lib.Context(0)
+    try:
+        v, ms, phases, labels = device_steps(ctx, x, eps, mp, a.p3_steps, 1)
+        st = phases[-1]
+        all_core = bool(np.all(labels == -1))
+        del labels
+        out = {"workload": "skewed_2d", "n": n, "d": 2, "eps": eps, "min_points": mp,
+               "gpu": {"value": round(v, 2), "unit": "Mpoints/s", "ms_per_step": round(ms, 3),
+                       "input": "device-resident"},
+               "phases_ms": {k: round(float(np.median([p[k] for p in phases])), 3)
+                             for k in ("ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union",
+                                       "ms_border", "ms_total")},
+               "sum_candidates": int(st["sum_candidates"]), "occupied_cells": int(st["occupied_cells"]),
+               "labels_all_minus_one": all_core}
+        if not a.no_cpu_baseline:
+            lo = np.partition(x[:, 0], n // 2)[n // 2]
+            hi = np.partition(x[:, 0], n // 2 + a.p3_cpu_sample)[n // 2 + a.p3_cpu_sample]
+            slab = np.ascontiguousarray(x[(x[:, 0] >= lo) & (x[:, 0] < hi)])
+            ctx.cluster(slab, eps, mp)
+            slab_cand = int(ctx.stats()["sum_candidates"])
+            threads = a.cpu_threads or cpu_threads()
+            print(f"[bench] skewed_2d: reference on a {len(slab)}-row slab, {threads} threads",
+                  file=sys.stderr, flush=True)
+            ref = reference_leg(slab, eps, mp, threads, 1,
+                                f"{len(slab)} rows of the skewed_2d workload with x0 between its "
+                                f"0.5 and {0.5 + len(slab) / n:.4f} quantiles (same density)")
+            ref.pop("labels", None)
+            if ref.get("value"):
+                t_slab = float(ref["times_s"][0])
+                t_full = t_slab * st["sum_candidates"] / max(1, slab_cand)
+                ref["slab_sum_candidates"] = slab_cand
+                ref["full_s_extrapolated"] = round(t_full, 1)
+                ref["full_value_extrapolated"] = round(n / t_full / 1e6, 6)
+                ref["note"] = ("a full run is infeasible (Σcand of the full workload / Σcand of "
+                               "the slab times the slab's time); the slab is thinner than eps, so "
+                               "its per-row work is far below the full workload's")
+                out["speedup_vs_cpu_extrapolated"] = round(v / ref["full_value_extrapolated"], 1)
+            out["cpu"] = ref
+    finally:
+        ctx.close()
+    del x
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +457,15 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=500_000,
                     help="rows in the CPU-baseline slab")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu-full", action="store_true",
+                    help="skip the reference's full-size run (about 100 s on 16 threads)")
+    ap.add_argument("--cpu-one-thread-sample", type=int, default=100_000,
+                    help="rows of the one-thread reference run")
+    ap.add_argument("--no-host-to-host", action="store_true")
+    ap.add_argument("--no-p1", action="store_true")
+    ap.add_argument("--no-p3", action="store_true")
+    ap.add_argument("--p3-steps", type=int, default=3)
+    ap.add_argument("--p3-cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -311,35 +490,15 @@ def main() -> None:
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
     ctx = _lib.Context(0)
-    xd = _lib.DeviceBuffer.from_array(ctx, x)
-    ld = _lib.DeviceBuffer(ctx, 8 * n)
-
-    def step():
-        ctx.cluster_device(xd.ptr, n, d, np.float32, eps, mp, ld.ptr)
-
     # phase times come from HIP events the library records on its own stream
     # (the one every kernel is launched on) during the timed steps themselves
-    ctx.set_timing(True)
-    for _ in range(a.warmup):
-        step()
-    ctx.synchronize()
-    phases = []
-    t_start = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-        phases.append(ctx.stats())
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ctx.set_timing(False)
-    ms_per_step = 1e3 * elapsed / a.steps
-    value = n * a.steps / elapsed / 1e6
-    print(f"[bench] {a.steps} steps in {elapsed:.3f}s: {ms_per_step:.2f} ms/step, "
-          f"{value:.1f} Mpoints/s", file=sys.stderr, flush=True)
+    value, ms_per_step, phases, labels = device_steps(ctx, x, eps, mp, a.steps, a.warmup)
+    print(f"[bench] {a.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} Mpoints/s",
+          file=sys.stderr, flush=True)
     st = phases[-1]
     keys = ["ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union", "ms_border", "ms_total",
             "ms_hook_kernel", "ms_union_kernel", "ms_border_kernel"]
     med = {k: float(np.median([p[k] for p in phases])) for k in keys}
-    labels = ld.to_array(np.int64, n)
     roofline = kernel_roofline(st, med, n, d, x.dtype.itemsize, a.workload)
     pair_ms = med["ms_core"] + med["ms_hook_kernel"] + med["ms_union_kernel"] + med["ms_border_kernel"]
 
@@ -370,44 +529,77 @@ def main() -> None:
                                           "ambiguous_borders", "sum_candidates", "boxes",
                                           "neighbor_rows", "open_box_pairs")},
     }
+    if not a.no_host_to_host:
+        result["host_to_host"] = host_to_host(ctx, x, eps, mp, 5)
+        print(f"[bench] host to host: {result['host_to_host']}", file=sys.stderr, flush=True)
 
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or cpu_threads()
+        result["host"] = host_info()
         sample = slab_sample(x, a.cpu_sample)
         print(f"[bench] cpu baseline: reference on {len(sample)} rows, {threads} threads",
               file=sys.stderr, flush=True)
-        res, ref_labels, err = run_reference(sample, eps, mp, threads, a.cpu_reps)
-        if res is None:
-            result["cpu_baseline"] = {"value": None, "error": err}
+        ref = reference_leg(sample, eps, mp, threads, a.cpu_reps,
+                            f"{len(sample)} rows of the {a.workload} workload with x0 below its "
+                            f"{len(sample) / n:.3f} quantile (same density); median of "
+                            f"{a.cpu_reps} runs of HPDBSCAN::cluster<float>(ptr, n, d, {threads})")
+        ref_labels = ref.pop("labels", None)
+        if ref.get("value") is None:
+            result["cpu_baseline"] = {"value": None, "error": ref.get("error")}
         else:
-            cpu_v = len(sample) / res["median_s"] / 1e6
+            cpu_v = ref["value"]
             result["cpu_baseline"] = {
-                "value": round(cpu_v, 4), "unit": "Mpoints/s", "cores": threads, "kind": "reference",
-                "sample": (f"{len(sample)} rows of the {a.workload} workload with x0 below its "
-                           f"{len(sample) / n:.3f} quantile (same density); median of "
-                           f"{a.cpu_reps} runs of HPDBSCAN::cluster<float>(ptr, n, d, {threads})"),
-                "times_s": res["times_s"], "cpu": cpu_model()}
+                "value": cpu_v, "unit": "Mpoints/s", "cores": threads, "kind": "reference",
+                "sample": ref["sample"], "times_s": ref["times_s"], "cpu": cpu_model()}
             gl = ctx.cluster(sample, eps, mp)
-            result["parity"] = {
-                "sample_rows": int(len(sample)),
-                "core_flags_equal": bool(np.array_equal(gl < 0, ref_labels < 0)),
-                "noise_equal": bool(np.array_equal(gl == 0, ref_labels == 0)),
-                "core_ids_equal": bool(np.array_equal(gl[gl < 0], ref_labels[gl < 0])),
-                "label_mismatches": int((gl != ref_labels).sum()),
-                "note": "reference run with several threads; only border rows next to two "
-                        "clusters may differ",
-            }
+            result["parity"] = dict(sample_rows=int(len(sample)), **parity_fields(gl, ref_labels),
+                                    note="reference run with several threads; only border rows "
+                                         "next to two clusters may differ")
             result["speedup_vs_cpu"] = round(value / cpu_v, 1) if cpu_v > 0 else None
+        small = slab_sample(x, a.cpu_one_thread_sample)
+        print(f"[bench] cpu baseline: reference on {len(small)} rows, 1 thread", file=sys.stderr,
+              flush=True)
+        ref1 = reference_leg(small, eps, mp, 1, 1,
+                             f"{len(small)} rows of the {a.workload} workload with x0 below its "
+                             f"{len(small) / n:.4f} quantile (same density), one thread")
+        ref1_labels = ref1.pop("labels", None)
+        if ref1_labels is not None:
+            ref1["parity_gpu_one_thread_rule"] = parity_fields(
+                _one_thread_labels(small, eps, mp), ref1_labels)
+        result["cpu_one_thread"] = ref1
+        if not a.no_cpu_full:
+            print(f"[bench] cpu baseline: reference on all {n} rows, {threads} threads (one run)",
+                  file=sys.stderr, flush=True)
+            full = reference_leg(x, eps, mp, threads, 1,
+                                 f"the full {a.workload} workload, one run")
+            full_labels = full.pop("labels", None)
+            if full_labels is not None:
+                full["parity"] = parity_fields(labels, full_labels)
+                full["speedup_vs_cpu"] = round(value / full["value"], 1)
+            result["cpu_full_size"] = full
 
-    xd.close()
-    ld.close()
     ctx.close()
     del labels
+    if not a.no_p1:
+        result["p1"] = p1_leg(a)
+    if not a.no_p3:
+        result["p3"] = p3_leg(a)
     if not a.no_batched:
         result["batched"] = batched_leg(a)
     if not a.no_serving:
         result["serving"] = serving_leg(a)
     print(json.dumps(result), flush=True)
+
+
+def _one_thread_labels(x, eps, mp):
+    """GPU labels under the one-thread border rule (every label comparable
+    with the reference run at one thread)."""
+    c = _lib.Context(0)
+    try:
+        c.set_border_rule(_lib.BORDER_ONE_THREAD)
+        return c.cluster(x, eps, mp)
+    finally:
+        c.close()
 
 
 if __name__ == "__main__":

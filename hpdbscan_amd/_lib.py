@@ -336,6 +336,13 @@ class DeviceBuffer:
         check(load().hpd_memcpy_to_device(ctx.handle, _vp(buf.ptr), a.ctypes.data, a.nbytes))
         return buf
 
+    def copy_from(self, a: np.ndarray) -> None:
+        """Copy a host array into the buffer (synchronous)."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        check(load().hpd_memcpy_to_device(self.ctx.handle, _vp(self.ptr), a.ctypes.data, a.nbytes))
+
     def to_array(self, dtype, count: int) -> np.ndarray:
         out = np.empty(count, dtype=dtype)
         check(load().hpd_memcpy_to_host(self.ctx.handle, out.ctypes.data, _vp(self.ptr),
