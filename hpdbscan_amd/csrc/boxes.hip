@@ -47,18 +47,32 @@ constexpr int kSmallTests = 512;
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
-// rows in slot order: coordinates (SoA); with it the bounding box of every
-// box (ordered keys, pre-filled with +inf / -inf keys): a box
-// inside the wave's 64 slots is stored by its first lane, one that crosses
-// the wave's edge is merged with atomics
+// Rows in slot order: coordinates (SoA); with it the bounding box of every
+// box (ordered keys).  A block of kThreads slots reduces its boxes in LDS
+// (box ids are consecutive in slot order, so a box's LDS entry is its id less
+// the block's first box): a box inside the block is stored by the block; the
+// block's first box when it started in an earlier block, and its last box
+// when it goes on past the block, leave the block's share in
+// part[(2 * block + 0 / 1) * 2D ..] for k_box_span.  No global atomics: a
+// box of a million rows would otherwise take thousands of them on one
+// address.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_gather_boxes(
     const T* __restrict__ pts, int64_t n, const uint32_t* __restrict__ inrow_of_slot,
     T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ box_of_slot,
-    const int32_t* __restrict__ bstart, OrdKey<T>* __restrict__ bbox) {
+    const int32_t* __restrict__ bstart, OrdKey<T>* __restrict__ bbox, OrdKey<T>* __restrict__ part) {
     using U = OrdKey<T>;
-    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    __shared__ U slo[kThreads][D];
+    __shared__ U shi[kThreads][D];
+    const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kThreads;
+    const int64_t s = blk0 + threadIdx.x;
     const bool valid = s < n;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        slo[threadIdx.x][j] = static_cast<U>(~U(0));
+        shi[threadIdx.x][j] = U(0);
+    }
+    const int32_t b_first = box_of_slot[blk0];  // the block's first box
     int32_t b = -1 - lane_id();  // distinct keys for lanes past the end
     T x[D];
     if (valid) {
@@ -73,25 +87,84 @@ __global__ __launch_bounds__(kThreads) void k_gather_boxes(
 #pragma unroll
         for (int j = 0; j < D; ++j) x[j] = T(0);
     }
-    const bool first = run_head(b);  // every lane takes part in the shuffles
-    const bool head = valid && first;
-    const int64_t w0 = s - lane_id();  // first slot of this wave
-    bool inside = false;
-    if (head) inside = bstart[b] >= w0 && bstart[b + 1] <= w0 + kWave;
+    __syncthreads();
+    // every lane takes part in the shuffles; the head lane of each run in the
+    // wave (lane 0 included) adds the run's extremes to the box's LDS entry
+    const bool head = valid && run_head(b);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         const U k = ord_key(x[j]);
         const U lo = run_suffix_min<U>(k, b);
         const U hi = run_suffix_max<U>(k, b);
         if (head) {
-            U* dst = &bbox[(static_cast<int64_t>(b) * D + j) * 2];
-            if (inside) {
-                dst[0] = lo;
-                dst[1] = hi;
-            } else {
-                atomicMin(&dst[0], lo);
-                atomicMax(&dst[1], hi);
-            }
+            atomicMin(&slo[b - b_first][j], lo);
+            atomicMax(&shi[b - b_first][j], hi);
+        }
+    }
+    __syncthreads();
+    const int64_t blk1 = blk0 + kThreads < n ? blk0 + kThreads : n;  // end of the block's slots
+    const int32_t nruns = box_of_slot[blk1 - 1] - b_first + 1;
+    const int t = threadIdx.x;
+    if (t < nruns) {
+        const int32_t bb = b_first + t;
+        const bool starts = bstart[bb] >= blk0;
+        const bool ends = bstart[bb + 1] <= blk1;
+        U* dst = starts && ends ? &bbox[static_cast<int64_t>(bb) * D * 2]
+                                : &part[(2 * static_cast<int64_t>(blockIdx.x) + (starts ? 1 : 0)) * D * 2];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            dst[2 * j] = slo[t][j];
+            dst[2 * j + 1] = shi[t][j];
+        }
+    }
+}
+
+// Bounding boxes of the boxes that span blocks of k_gather_boxes: one wave
+// per block; the wave of the block where such a box starts combines the
+// block's share (its last run) with the shares of the following blocks (their
+// first runs) up to the box's last block.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_span(int64_t n, int64_t nblocks,
+                                                       const int32_t* __restrict__ box_of_slot,
+                                                       const int32_t* __restrict__ bstart,
+                                                       const OrdKey<T>* __restrict__ part,
+                                                       OrdKey<T>* __restrict__ bbox) {
+    using U = OrdKey<T>;
+    const int64_t blk = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (blk >= nblocks) return;
+    const int64_t blk0 = blk * kThreads;
+    const int64_t blk1 = blk0 + kThreads < n ? blk0 + kThreads : n;
+    const int32_t b = box_of_slot[blk1 - 1];
+    const int64_t b0 = bstart[b], b1 = bstart[b + 1];
+    if (b0 < blk0 || b1 <= blk1) return;  // not a box that starts here and goes on
+    const int64_t last = (b1 - 1) / kThreads;
+    U lo[D], hi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        lo[j] = static_cast<U>(~U(0));
+        hi[j] = U(0);
+    }
+    for (int64_t k = blk + lane_id(); k <= last; k += kWave) {
+        const U* src = &part[(2 * k + (k == blk ? 1 : 0)) * D * 2];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            lo[j] = min(lo[j], src[2 * j]);
+            hi[j] = max(hi[j], src[2 * j + 1]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[j] = min(lo[j], static_cast<U>(__shfl_xor(lo[j], o, 64)));
+            hi[j] = max(hi[j], static_cast<U>(__shfl_xor(hi[j], o, 64)));
+        }
+    }
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            bbox[(static_cast<int64_t>(b) * D + j) * 2] = lo[j];
+            bbox[(static_cast<int64_t>(b) * D + j) * 2 + 1] = hi[j];
         }
     }
 }
@@ -799,12 +872,22 @@ int box_sub_bits(int d) {
 template <typename T, int D>
 void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot, T* soa,
                          int64_t ld, const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
-                         hipStream_t s) {
+                         void* part, hipStream_t s) {
     using U = OrdKey<T>;
     if (n <= 0) return;
-    hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                       pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, static_cast<U*>(bbox));
+    const int64_t nb = (n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, s,
+                       pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, static_cast<U*>(bbox),
+                       static_cast<U*>(part));
+    hipLaunchKernelGGL((k_box_span<T, D>), dim3(blocks_for(nb, kWavesPerBlock)), dim3(kThreads), 0,
+                       s, n, nb, box_of_slot, bstart, static_cast<const U*>(part),
+                       static_cast<U*>(bbox));
     HPD_HIP(hipGetLastError());
+}
+
+size_t gather_part_bytes(int64_t n, int d, int value_bytes) {
+    const size_t nb = static_cast<size_t>((n + kThreads - 1) / kThreads);
+    return 2 * nb * 2 * static_cast<size_t>(d) * static_cast<size_t>(value_bytes);
 }
 
 template <typename K>
@@ -1007,7 +1090,8 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
 
 #define HPD_INST_BOX_KEYS(T, D)                                                                   \
     template void launch_gather_boxes<T, D>(const T*, int64_t, const uint32_t*, T*, int64_t,      \
-                                            const int32_t*, const int32_t*, void*, hipStream_t);  \
+                                            const int32_t*, const int32_t*, void*, void*,         \
+                                            hipStream_t);                                         \
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \

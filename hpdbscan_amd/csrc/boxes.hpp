@@ -30,11 +30,13 @@ struct BoxArgs {
 };
 
 // coordinates in slot order (soa[j * ld + s] = pts[inrow_of_slot[s] * D + j])
-// and the bounding boxes (bbox, initialised by launch_box_table)
+// and the bounding box of every box (bbox, 2 * D ordered keys per box, all
+// written); part holds gather_part_bytes(n, D, sizeof(T)) bytes of scratch
 template <typename T, int D>
 void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot, T* soa,
                          int64_t ld, const int32_t* box_of_slot, const int32_t* bstart, void* bbox,
-                         hipStream_t s);
+                         void* part, hipStream_t s);
+size_t gather_part_bytes(int64_t n, int d, int value_bytes);
 
 // Boxes of keys sorted by (cell, sub-box digit): runs of equal keys.
 // cflags / cscan are the cell-run flags and their exclusive scan (they give

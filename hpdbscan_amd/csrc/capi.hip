@@ -322,6 +322,7 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     b += L + 3 * 4 * L + 3 * 4 * nn;           // core, parent, rsnap, rootslot, cmin/cmax/rcenter
     b += 8 * nn + 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // scratch64, positions, box tables
     b += 2 * D * 8 * nn;                       // box bounding boxes
+    b += gather_part_bytes(n, D, static_cast<int>(sizeof(T)));  // their per-block shares
     const size_t nj = static_cast<size_t>(njobs);
     const size_t mm_blocks = nn / static_cast<size_t>(minmax_rows_per_block()) + nj + 1;
     b += sizeof(T) * 2 * D * (mm_blocks + nj) + 4 * (mm_blocks + nj + 2) + 64;  // min/max
@@ -415,11 +416,13 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         bcluster = A.take_n<int32_t>(nn);
         btight = A.take_n<uint8_t>(nn);
         bbox = A.take(2 * D * sizeof(T) * nn);
+        void* bpart = A.take(gather_part_bytes(n, D, static_cast<int>(sizeof(T))));
         // box flags and their scan in the sort's free buffers
         launch_box_table<K>(keys, n, flags, flags + n, vals_free,
                             reinterpret_cast<uint32_t*>(keys_free), tmp, bstart, bcell, box_of_slot,
-                            &counters[3], bbox, 2 * D, static_cast<int>(sizeof(T)), s);
-        launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, s);
+                            &counters[3], nullptr, 2 * D, static_cast<int>(sizeof(T)), s);
+        launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, bpart,
+                                  s);
     } else {
         launch_gather_soa<T, D>(pts, n, inrow_of_slot, soa, ld, s);
     }

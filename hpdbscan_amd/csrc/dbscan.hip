@@ -165,7 +165,9 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
     load_row<T, D>(a.soa, a.ld, valid ? p : c.first, pc);
     uint32_t cnt = 0;
     // rows of a tight box with >= minPts rows are core without counting
-    if (valid && (a.btight[a.box_of_slot[p]] & 2)) cnt = min_pts;
+    const int32_t bp = valid ? a.box_of_slot[p] : -1 - lane_id();
+    const bool full = valid && (a.btight[bp] & 2);
+    if (full) cnt = min_pts;
     bool need = valid && cnt < min_pts;
     if (__any(need)) {
         for_each_nbr_box<T, D>(a, c, nullptr, [&](int32_t, const double* lo, const double* hi,
@@ -192,16 +194,22 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
     if (valid) core[p] = is_core ? 1 : 0;
     (void)ncore;
     if (a.bmincore) {
-        // smallest core input row per box: a box inside this chunk is stored
-        // by its first lane, one that extends past it is merged atomically
-        const int32_t b = valid ? a.box_of_slot[p] : -1 - lane_id();
-        const int32_t m = run_suffix_min<int32_t>(is_core ? static_cast<int32_t>(a.inrow[p]) : INT_MAX, b);
-        const bool first = run_head(b);  // every lane takes part in the shuffles
+        // smallest core input row per box.  A box holds its rows in input
+        // order, and every row of a tight box with >= minPts rows is core:
+        // its value is its first slot's input row, stored by that lane (no
+        // atomics on one address from every chunk of a large box).  Other
+        // boxes: one inside this chunk is stored by its first lane, one that
+        // extends past it is merged atomically.
+        const uint32_t row = valid ? a.inrow[p] : 0u;
+        if (full && a.bstart[bp] == p) a.bmincore[bp] = static_cast<int32_t>(row);
+        const int32_t m =
+            run_suffix_min<int32_t>(is_core && !full ? static_cast<int32_t>(row) : INT_MAX, bp);
+        const bool first = run_head(bp);  // every lane takes part in the shuffles
         if (valid && first && m != INT_MAX) {
-            if (a.bstart[b] >= c.first && a.bstart[b + 1] <= c.first + kWave)
-                a.bmincore[b] = m;
+            if (a.bstart[bp] >= c.first && a.bstart[bp + 1] <= c.first + kWave)
+                a.bmincore[bp] = m;
             else
-                atomicMin(&a.bmincore[b], m);
+                atomicMin(&a.bmincore[bp], m);
         }
     }
 }
