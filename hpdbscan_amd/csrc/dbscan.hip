@@ -170,25 +170,57 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
     if (full) cnt = min_pts;
     bool need = valid && cnt < min_pts;
     if (__any(need)) {
-        for_each_nbr_box<T, D>(a, c, nullptr, [&](int32_t, const double* lo, const double* hi,
-                                                  int32_t q0, int32_t q1, int32_t) {
-            double gmin2, gmax2;
-            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
-            const bool near = need && gmin2 <= gap2_max;
-            if (!__any(near)) return false;
-            const bool whole = near && gmax2 <= full2;
-            if (whole) cnt += static_cast<uint32_t>(q1 - q0);
-            const bool part = near && !whole;
-            if (__any(part)) {
-                auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
-                    cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
-                };
-                auto stop = [&] { return static_cast<bool>(__all(!part || cnt >= min_pts)); };
-                visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, q0, q1, body, stop);
-            }
-            need = valid && cnt < min_pts;
-            return !__any(need);
-        }, true);
+        // the group's boxes are first classified against the bounding box of
+        // the rows still counting, one box per lane: a box farther than eps
+        // from all of them is skipped, one within eps of all of them counts
+        // whole for every row (both exactly as the per-row tests below would
+        // decide, box_box bounding them monotonically); only the others are
+        // walked box by box
+        double clo[D], chi[D];
+        rows_bbox<T, D>(pc, need, clo, chi);
+        for_each_row_box_group<T, D>(
+            c.r0, c.r1, uniform(a.rcenter[c.u]), a.rcells, a.cbox, a.bstart,
+            static_cast<const OrdKey<T>*>(a.bbox), nullptr,
+            [&](int32_t nk, int32_t bl, const double* lol, const double* hil, int32_t q0l,
+                int32_t q1l, int32_t) {
+                double cmin2, cmax2;
+                box_box<D>(clo, chi, lol, hil, cmin2, cmax2);
+                const bool mine = lane_id() < nk;
+                const bool all_in = mine && cmax2 <= full2;
+                const bool some = mine && !all_in && cmin2 <= gap2_max;
+                uint32_t add = all_in ? static_cast<uint32_t>(q1l - q0l) : 0u;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
+                if (need) cnt += add;
+                need = valid && cnt < min_pts;
+                for (uint64_t todo = ballot(some); todo != 0 && __any(need); todo &= todo - 1) {
+                    const int j = static_cast<int>(__builtin_ctzll(todo));
+                    double lo[D], hi[D];
+#pragma unroll
+                    for (int e = 0; e < D; ++e) {
+                        lo[e] = bcast(lol[e], j);
+                        hi[e] = bcast(hil[e], j);
+                    }
+                    const int32_t q0 = bcast(q0l, j), q1 = bcast(q1l, j);
+                    double gmin2, gmax2;
+                    point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+                    const bool near = need && gmin2 <= gap2_max;
+                    if (!__any(near)) continue;
+                    const bool whole = near && gmax2 <= full2;
+                    if (whole) cnt += static_cast<uint32_t>(q1 - q0);
+                    const bool part = near && !whole;
+                    if (__any(part)) {
+                        auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
+                            cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
+                        };
+                        auto stop = [&] { return static_cast<bool>(__all(!part || cnt >= min_pts)); };
+                        visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, q0, q1, body, stop);
+                    }
+                    need = valid && cnt < min_pts;
+                }
+                return !__any(need);
+            },
+            INT_MAX, uniform(a.cbox[c.u]));
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;

@@ -221,6 +221,113 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
     }
 }
 
+// The same visit as for_each_row_box, handed to fn a group of up to 64 boxes
+// at a time with one box per lane: fn(nk, bl, lol, hil, q0l, q1l, tagl) gets
+// the group's size nk (wave-uniform) and, on lanes below nk, the lane's box,
+// its bounding box, rows and tag (lanes at or above nk repeat the group's
+// last box); it returns true to stop the visit.  Boxes keep the visiting
+// order of for_each_row_box (lane order inside a group).  Lets a kernel
+// classify a whole group against the bounding box of its rows in parallel
+// and walk only the boxes that need row tests.
+template <typename T, int D, typename Fn>
+__device__ __forceinline__ void for_each_row_box_group(int32_t r0, int32_t r1, int32_t center,
+                                                       const int2* __restrict__ rcells,
+                                                       const int32_t* __restrict__ cbox,
+                                                       const int32_t* __restrict__ bstart,
+                                                       const OrdKey<T>* __restrict__ bbox,
+                                                       const int32_t* __restrict__ tag, Fn&& fn,
+                                                       int32_t bmax = INT_MAX,
+                                                       int32_t first = -1) {
+    const int32_t nrows = r1 - r0;
+    const int lane = __lane_id();
+    for (int32_t g0 = 0; g0 < nrows; g0 += 64) {
+        const int32_t i = g0 + lane;  // visiting order: own row, then the others
+        int32_t rb0 = 0, rcnt = 0;
+        if (i < nrows) {
+            const int32_t r = i == 0 ? center : (r0 + i - 1 < center ? r0 + i - 1 : r0 + i);
+            const int2 cs = rcells[r];
+            rb0 = cbox[cs.x];
+            rcnt = max(0, min(cbox[cs.y], bmax) - rb0);  // boxes below bmax only
+        }
+        int32_t incl = rcnt;  // inclusive prefix over the lanes
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        const int32_t excl = incl - rcnt;
+        const int32_t total = bcast(incl, 63);
+        const int32_t grows = min(64, nrows - g0);
+        int32_t rot = 0, rot_n = 0;
+        if (g0 == 0 && first >= 0) {
+            const int32_t b0 = bcast(rb0, 0);
+            rot_n = bcast(rcnt, 0);
+            rot = (first >= b0 && first < b0 + rot_n) ? first - b0 : 0;
+        }
+        for (int32_t k0 = 0; k0 < total; k0 += 64) {
+            const int32_t kk = min(k0 + lane, total - 1);
+            int32_t row = 0;  // last row whose first box index is <= kk
+            for (int32_t t = 1; t < grows; ++t) row = bcast(excl, t) <= kk ? t : row;
+            int32_t local = kk - __shfl(excl, row, 64);
+            if (row == 0 && rot > 0) local = local + rot < rot_n ? local + rot : local + rot - rot_n;
+            const int32_t bl = __shfl(rb0, row, 64) + local;
+            double lol[D], hil[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                lol[j] = box_lo<T, D>(bbox, bl, j);
+                hil[j] = box_hi<T, D>(bbox, bl, j);
+            }
+            const int32_t q0l = bstart[bl], q1l = bstart[bl + 1];
+            const int32_t tagl = tag ? tag[bl] : 0;
+            if (fn(min(64, total - k0), bl, lol, hil, q0l, q1l, tagl)) return;
+        }
+    }
+}
+
+// Box-to-box squared distances (double): gmin2 between the nearest points
+// of [alo, ahi] and [blo, bhi], gmax2 between the farthest.  For every row p
+// inside box a, gmin2 <= gmin2(p, b) and gmax2(p, b) <= gmax2, also as
+// computed in double: every step (subtract, max, square, add) rounds
+// monotonically.
+template <int D>
+__device__ __forceinline__ void box_box(const double* alo, const double* ahi, const double* blo,
+                                        const double* bhi, double& gmin2, double& gmax2) {
+    gmin2 = 0.0;
+    gmax2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double g = fmax(0.0, fmax(blo[j] - ahi[j], alo[j] - bhi[j]));
+        const double f = fmax(bhi[j] - alo[j], ahi[j] - blo[j]);
+        gmin2 += g * g;
+        gmax2 += f * f;
+    }
+}
+
+// A wave-uniform double held in scalar registers (v_readfirstlane).
+__device__ __forceinline__ double uniform_d(double v) {
+    const long long x = __double_as_longlong(v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// Wave-uniform bounding box (double) of the rows of the lanes where `on`.
+template <typename T, int D>
+__device__ __forceinline__ void rows_bbox(const T* pc, bool on, double* lo, double* hi) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double a = on ? static_cast<double>(pc[j]) : __builtin_huge_val();
+        double b = on ? static_cast<double>(pc[j]) : -__builtin_huge_val();
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a = fmin(a, __shfl_xor(a, o, 64));
+            b = fmax(b, __shfl_xor(b, o, 64));
+        }
+        lo[j] = uniform_d(a);
+        hi[j] = uniform_d(b);
+    }
+}
+
 // Runs of equal `key` over consecutive lanes (boxes are contiguous slot
 // ranges): afterwards every lane holds the minimum (maximum) of v over the
 // lanes from itself to the end of its run.
