@@ -671,6 +671,39 @@ __global__ __launch_bounds__(kThreads) void k_select_items(int32_t nitems,
         [&](int64_t i, unsigned int pos) { out[pos] = items[i]; });
 }
 
+__global__ __launch_bounds__(kThreads) void k_core_full(int64_t n,
+                                                        const int32_t* __restrict__ box_of_slot,
+                                                        const uint8_t* __restrict__ btight,
+                                                        const int32_t* __restrict__ bstart,
+                                                        const uint32_t* __restrict__ inrow,
+                                                        uint8_t* __restrict__ core,
+                                                        int32_t* __restrict__ bmincore) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (s >= n) return;
+    const int32_t b = box_of_slot[s];
+    const bool full = (btight[b] & 2) != 0;
+    core[s] = full ? 1 : 0;
+    if (full && bstart[b] == s) bmincore[b] = static_cast<int32_t>(inrow[s]);
+}
+
+// the chunks holding a row of a box that is not tight with >= minPts rows
+__global__ __launch_bounds__(kThreads) void k_select_core_chunks(
+    int32_t nitems, const int2* __restrict__ items, const int32_t* __restrict__ ustart,
+    const int32_t* __restrict__ box_of_slot, const uint8_t* __restrict__ btight,
+    int2* __restrict__ out, unsigned int* __restrict__ count) {
+    block_select(
+        nitems, count,
+        [&](int64_t i) {
+            const int2 it = items[i];
+            const int32_t last = min(it.y + kWave, ustart[it.x + 1]) - 1;
+            const int32_t b1 = box_of_slot[last];
+            for (int32_t b = box_of_slot[it.y]; b <= b1; ++b)
+                if (!(btight[b] & 2)) return true;
+            return false;
+        },
+        [&](int64_t i, unsigned int pos) { out[pos] = items[i]; });
+}
+
 // root of every box's hook tree -> rsnap (-1 for boxes without cores)
 __global__ __launch_bounds__(kThreads) void k_box_roots(int32_t nbox,
                                                         const int32_t* __restrict__ bmincore,
@@ -981,6 +1014,24 @@ void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list
     if (nbox <= 0) return;
     hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox, kSelectSpan)), dim3(kThreads), 0, s,
                        nbox, mode, v, list, count);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btight,
+                      const int32_t* bstart, const uint32_t* inrow, uint8_t* core,
+                      int32_t* bmincore, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_core_full, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n,
+                       box_of_slot, btight, bstart, inrow, core, bmincore);
+    HPD_HIP(hipGetLastError());
+}
+
+void launch_select_core_chunks(int32_t nitems, const int2* items, const int32_t* ustart,
+                               const int32_t* box_of_slot, const uint8_t* btight, int2* out,
+                               unsigned int* count, hipStream_t s) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(k_select_core_chunks, dim3(blocks_for(nitems, kSelectSpan)), dim3(kThreads),
+                       0, s, nitems, items, ustart, box_of_slot, btight, out, count);
     HPD_HIP(hipGetLastError());
 }
 

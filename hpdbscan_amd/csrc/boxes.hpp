@@ -90,6 +90,17 @@ void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2*
                           const int32_t* rsnap, const uint8_t* core, const int32_t* blist,
                           int32_t nlist, int32_t* bparent, hipStream_t s);
 // the box chunks (items) of the boxes with open pairs (off[b+1] > off[b])
+// core[s] = 1 for the rows of tight boxes with >= minPts rows (btight bit 1),
+// 0 for every other row; for those boxes bmincore[b] = inrow[bstart[b]]
+// (every row is core and a box holds its rows in input order)
+void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btight,
+                      const int32_t* bstart, const uint32_t* inrow, uint8_t* core,
+                      int32_t* bmincore, hipStream_t s);
+// out[..] = the chunks (cell, first row) of items holding a row outside
+// those boxes; *count += their number
+void launch_select_core_chunks(int32_t nitems, const int2* items, const int32_t* ustart,
+                               const int32_t* box_of_slot, const uint8_t* btight, int2* out,
+                               unsigned int* count, hipStream_t s);
 void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
                          unsigned int* count, hipStream_t s);
 template <typename T, int D>

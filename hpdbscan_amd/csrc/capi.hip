@@ -492,7 +492,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                     align_up(static_cast<size_t>(nditems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nsitems) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nbitems) * sizeof(int2), 256) +
-                    align_up(static_cast<size_t>(ncitems) * sizeof(int2), 256) + 2048);
+                    2 * align_up(static_cast<size_t>(ncitems) * sizeof(int2), 256) + 2048);
     int2* ranges = X.take_n<int2>(total_ranges);
     int2* rcells = X.take_n<int2>(total_ranges);
     int2* items = X.take_n<int2>(static_cast<size_t>(nitems));
@@ -501,6 +501,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int2* sitems = boxes ? X.take_n<int2>(static_cast<size_t>(nsitems)) : nullptr;
     int2* uitems = boxes ? X.take_n<int2>(static_cast<size_t>(nbitems)) : nullptr;
     int2* citems = boxes ? X.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
+    int2* citems_need = boxes ? X.take_n<int2>(static_cast<size_t>(ncitems)) : nullptr;
     if (stash)
         launch_neighbor_copy(stash, stash_rows, ustart, m, nranges, ranges, rcells, rcenter,
                              by_class, s);
@@ -562,6 +563,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         a.nditems = nditems;
         a.citems = citems;
         a.ncitems = ncitems;
+        a.citems_need = citems_need;
+        a.ncitems_need_dev = &counters[12];
         a.sitems = sitems;
         a.nsitems = nsitems;
     }

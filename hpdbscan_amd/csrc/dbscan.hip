@@ -36,6 +36,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "boxes.hpp"
 #include "common.hpp"
 #include "dbscan.hpp"
 #include "grid.hpp"
@@ -63,10 +64,12 @@ struct Chunk {
 
 __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int32_t nitems,
                                             const int32_t* __restrict__ ustart,
-                                            const uint32_t* __restrict__ roff, Chunk& c) {
+                                            const uint32_t* __restrict__ roff, Chunk& c,
+                                            const unsigned int* nitems_dev = nullptr) {
     const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                 static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return false;
+    if (nitems_dev && wid >= uniform(static_cast<int32_t>(*nitems_dev))) return false;
     const int2 it = items[wid];
     c.u = uniform(it.x);
     c.first = uniform(it.y);
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
                                                          uint8_t* __restrict__ core,
                                                          unsigned long long* __restrict__ ncore) {
     Chunk c;
-    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c)) return;
+    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c, a.nditems_dev)) return;
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const uint32_t min_pts = jp.min_pts;
@@ -935,8 +938,19 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
                 ac.nditems = a.citems ? a.ncitems : a.nitems;
                 ac.nsitems = 0;
             }
-            if (a.citems && !a.sparse_core)  // rows of the other cells are not core
+            if (a.citems && !a.sparse_core && a.citems_need && a.bmincore) {
+                // rows of tight boxes with >= minPts rows are core, every other
+                // row starts non-core (no row of a cell outside citems is
+                // core); only the chunks holding another row are counted
+                launch_core_full(a.n, a.box_of_slot, a.btight, a.bstart, a.inrow, core, a.bmincore,
+                                 s);
+                launch_select_core_chunks(a.ncitems, a.citems, a.ustart, a.box_of_slot, a.btight,
+                                          a.citems_need, a.ncitems_need_dev, s);
+                ac.ditems = a.citems_need;
+                ac.nditems_dev = a.ncitems_need_dev;
+            } else if (a.citems && !a.sparse_core) {  // rows of the other cells are not core
                 HPD_HIP(hipMemsetAsync(core, 0, static_cast<size_t>(a.n), s));
+            }
             if (ac.nditems > 0)
                 hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(ac.nditems, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, ac, core, ncore);

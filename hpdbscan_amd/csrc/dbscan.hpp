@@ -49,6 +49,14 @@ struct DbscanArgs {
     // hold cores (core flags of the others stay 0; null: every chunk)
     const int2* citems = nullptr;
     int32_t ncitems = 0;
+    // box mode without the core split: scratch for the chunks of citems
+    // holding a row outside the tight boxes of >= minPts rows (those rows
+    // are core without counting), and a device counter (zeroed) for them
+    int2* citems_need = nullptr;
+    unsigned int* ncitems_need_dev = nullptr;
+    // when not null, the number of work items of ditems is read from here
+    // (nditems is then only the grid's bound)
+    const unsigned int* nditems_dev = nullptr;
     // box mode: the core count also writes the smallest input row of a core
     // row per box (pre-filled with INT_MAX; stays INT_MAX without cores)
     int32_t* bmincore = nullptr;
