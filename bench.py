@@ -24,7 +24,9 @@ The JSON line also carries
                 hpd_cluster_batch_device call vs a loop of one call per
                 cloud: jobs/s of both, per-call p50/p99 of the loop, and
                 equality of the labels; the work split of both comes from
-                hpdbscan_amd.autotune run on the batch first
+                hpdbscan_amd.autotune run on the batch first; graph_* is the
+                same batch through the static pipeline captured once in a
+                hipGraph and replayed (hpd_context_set_graph)
   serving       a Poisson stream (200 requests/s, 10k-5M rows, d in 2..4)
                 through the continuous-batching server: offered vs achieved
                 load, p50/p99 latency
@@ -205,6 +207,36 @@ def batched_leg(a) -> dict:
         reps.append(time.perf_counter() - t)
     batched_labels = ld.to_array(np.int64, len(rows))
     t_batch = float(np.median(reps))
+    # the same batch through the static pipeline: captured in a hipGraph by
+    # the first call, replayed by the others (no host read-back in between)
+    graph = {}
+    if a.graph:
+        gctx = _lib.Context(0)
+        gctx.set_work_split(*ctx.work_split())
+        gctx.set_graph(True)
+        gl = _lib.DeviceBuffer(gctx, 8 * len(rows))
+        t = time.perf_counter()
+        gctx.cluster_batch_device(xd.ptr, off, 3, np.float32, eps, mps, gl.ptr)
+        gctx.synchronize()
+        t_capture = time.perf_counter() - t
+        greps = []
+        for _ in range(a.batch_reps):
+            t = time.perf_counter()
+            gctx.cluster_batch_device(xd.ptr, off, 3, np.float32, eps, mps, gl.ptr)
+            gctx.synchronize()
+            greps.append(time.perf_counter() - t)
+        g_labels = gl.to_array(np.int64, len(rows))
+        t_graph = float(np.median(greps))
+        graph = {"graph_jobs_per_s": round(nj / t_graph, 1), "graph_s": round(t_graph, 4),
+                 "graph_first_call_s": round(t_capture, 4),
+                 "graph_labels_equal": bool(np.array_equal(g_labels, batched_labels)),
+                 "graph_stats": gctx.graph_stats(),
+                 "graph_note": "static pipeline (worst-case tables, device-side counts and grids, "
+                               "no host read-back) captured in a hipGraph by the first call and "
+                               "replayed"}
+        gl.close()
+        gctx.close()
+        del g_labels
     # the plain loop: one hpd_cluster_device per cloud, same device buffers
     lat = []
     for j in range(nj):
@@ -234,6 +266,7 @@ def batched_leg(a) -> dict:
         "work_split": tuned["best"] if tuned else dict(zip(("sparse_max", "sparse_core"),
                                                            (8, False))),
         "note": "device-resident rows; a batched job's latency is the whole batch's",
+        **graph,
     }
 
 
@@ -469,6 +502,8 @@ def main() -> None:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="batched leg: skip the hipGraph-captured static pipeline")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="batched leg: library default work split instead of the autotuned one")
     ap.add_argument("--no-serving", action="store_true")

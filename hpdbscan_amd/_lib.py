@@ -113,6 +113,10 @@ SIGNATURES = {
     "hpd_context_set_work_split": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int]),
     "hpd_context_get_work_split": (ctypes.c_int, [_vp, _vp, _vp]),
     "hpd_context_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hpd_context_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hpd_context_get_graph_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64),
+                                                   ctypes.POINTER(ctypes.c_uint64),
+                                                   ctypes.POINTER(ctypes.c_uint64)]),
     "hpd_context_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(RunStats)]),
     "hpd_cluster": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.c_float, ctypes.c_uint64, _vp]),
@@ -225,6 +229,16 @@ class Context:
 
     def set_timing(self, enabled: bool) -> None:
         check(load().hpd_context_set_timing(self._h, 1 if enabled else 0))
+
+    def set_graph(self, enabled: bool) -> None:
+        """Graph mode: floating-point runs take the static pipeline, captured
+        in a hipGraph once per shape and replayed (same labels)."""
+        check(load().hpd_context_set_graph(self._h, 1 if enabled else 0))
+
+    def graph_stats(self) -> dict:
+        v = [ctypes.c_uint64() for _ in range(3)]
+        check(load().hpd_context_get_graph_stats(self._h, *(ctypes.byref(x) for x in v)))
+        return dict(zip(("launches", "captures", "fallbacks"), (int(x.value) for x in v)))
 
     def synchronize(self) -> None:
         check(load().hpd_context_synchronize(self._h))

@@ -213,8 +213,10 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
 
 // first box of every cell; cbox[m] = number of boxes
 __global__ __launch_bounds__(kThreads) void k_cell_boxes(const int32_t* __restrict__ bcell,
-                                                         int32_t nbox, int32_t m,
+                                                         Count nbox_c, Count m_c,
                                                          int32_t* __restrict__ cbox) {
+    const int32_t m = static_cast<int32_t>(m_c.get());
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b < nbox && (b == 0 || bcell[b] != bcell[b - 1])) cbox[bcell[b]] = b;
     if (b == 0) cbox[m] = nbox;
@@ -242,13 +244,14 @@ __device__ __forceinline__ U wave_max_u(U v) {
 // tight flag per box: dist2(extent) <= EPS2 in the reference's order
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restrict__ bbox,
-                                                        int32_t nbox,
+                                                        Count nbox_c,
                                                         const int32_t* __restrict__ bstart,
                                                         const int32_t* __restrict__ bcell,
                                                         const int32_t* __restrict__ job_of_cell,
                                                         const JobParams<T>* __restrict__ jobs,
                                                         uint8_t* __restrict__ btight,
                                                         unsigned int* __restrict__ nloose) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox) return;
     const JobParams<T>& jp = jobs[job_of_cell[bcell[b]]];
@@ -274,59 +277,64 @@ __global__ __launch_bounds__(kThreads) void k_box_tight(const OrdKey<T>* __restr
 // connected sets of boxes.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_hook(
-    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nitems,
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, Count nitems_c,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
     const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
     const int32_t* __restrict__ rcenter, int32_t* __restrict__ bparent) {
-    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                                static_cast<int32_t>(threadIdx.x >> 6));
-    if (wid >= nitems) return;
-    // boxes with cores; one hook per box is enough: its first chunk looks
-    // for it (a missed link is found by the exact union later)
-    const int32_t b = uniform(blist[wid]);
-    const int2 it = make_int2(b, uniform(bstart[b]));
-    if (uniform(bstart[b + 1]) - it.y <= kSmallBox) return;  // k_box_hook_small
-    const int32_t end = uniform(bstart[b + 1]);
-    const int32_t p = it.y + lane_id();
-    const bool pcore = p < end && core[p];
-    if (!__any(pcore)) return;
-    const int32_t u = uniform(bcell[b]);
-    const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
-    const T eps2 = jp.eps2;
-    const double gap2_max = jp.gap2_max;
-    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
-    T pc[D];
-    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
-    const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
-    const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
-    // the cell's own row first, from the own cell's first box: the box's
-    // nearest neighbours are there, so the first hit comes soonest; boxes
-    // before b only
-    for_each_row_box<T, D>(
-        r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, bmincore,
-        [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
-            int32_t mincore) {
-            if (mincore == INT_MAX) return false;  // no cores
-            double gmin2, gmax2;
-            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
-            const bool act = pcore && gmin2 <= gap2_max;
-            if (!__any(act)) return false;
-            // a core row within eps of the whole box is within eps of its cores
-            bool hit = act && gmax2 <= full2;
-            if (!__any(hit)) {
-                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-                    if (cq && act && within<D>(pc, qc, eps2)) hit = true;
-                };
-                auto stop = [&] { return static_cast<bool>(__any(hit)); };
-                if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
-            }
-            if (lane_id() == 0) bparent[b] = b2;
-            return true;
-        },
-        b, uniform(cbox[u]));
+    const int32_t nitems = static_cast<int32_t>(nitems_c.get());
+    const int32_t stride = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
+    // waves walk the items wid, wid + stride, ... (one each when the grid
+    // covers them)
+    for (int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                               static_cast<int32_t>(threadIdx.x >> 6));
+         wid < nitems; wid += stride) [&] {
+        // boxes with cores; one hook per box is enough: its first chunk looks
+        // for it (a missed link is found by the exact union later)
+        const int32_t b = uniform(blist[wid]);
+        const int2 it = make_int2(b, uniform(bstart[b]));
+        if (uniform(bstart[b + 1]) - it.y <= kSmallBox) return;  // k_box_hook_small
+        const int32_t end = uniform(bstart[b + 1]);
+        const int32_t p = it.y + lane_id();
+        const bool pcore = p < end && core[p];
+        if (!__any(pcore)) return;
+        const int32_t u = uniform(bcell[b]);
+        const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
+        const T eps2 = jp.eps2;
+        const double gap2_max = jp.gap2_max;
+        const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+        T pc[D];
+        load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+        const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
+        const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
+        // the cell's own row first, from the own cell's first box: the box's
+        // nearest neighbours are there, so the first hit comes soonest; boxes
+        // before b only
+        for_each_row_box<T, D>(
+            r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, bmincore,
+            [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
+                int32_t mincore) {
+                if (mincore == INT_MAX) return false;  // no cores
+                double gmin2, gmax2;
+                point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+                const bool act = pcore && gmin2 <= gap2_max;
+                if (!__any(act)) return false;
+                // a core row within eps of the whole box is within eps of its cores
+                bool hit = act && gmax2 <= full2;
+                if (!__any(hit)) {
+                    auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                        if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+                    };
+                    auto stop = [&] { return static_cast<bool>(__any(hit)); };
+                    if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+                }
+                if (lane_id() == 0) bparent[b] = b2;
+                return true;
+            },
+            b, uniform(cbox[u]));
+    }();
 }
 
 // Root join: every root of the hook forest (a box without an earlier
@@ -336,49 +344,54 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
 // a cluster, so far fewer pairs stay open for the union.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_root_join(
-    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nitems,
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, Count nitems_c,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
     const int32_t* __restrict__ rsnap, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
     const int32_t* __restrict__ rcenter, int32_t* bparent) {
-    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                                static_cast<int32_t>(threadIdx.x >> 6));
-    if (wid >= nitems) return;
-    // roots of the forest, each searched from its first chunk
-    const int32_t b = uniform(blist[wid]);
-    const int2 it = make_int2(b, uniform(bstart[b]));
-    const int32_t end = uniform(bstart[b + 1]);
-    const int32_t p = it.y + lane_id();
-    const bool pcore = p < end && core[p];
-    if (!__any(pcore)) return;
-    const int32_t u = uniform(bcell[b]);
-    const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
-    const T eps2 = jp.eps2;
-    const double gap2_max = jp.gap2_max;
-    T pc[D];
-    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
-    const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
-    const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
-    for_each_row_box<T, D>(
-        r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, rsnap,
-        [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
-            int32_t s2) {
-            if (s2 < 0 || s2 == b) return false;  // no cores, or b's own tree
-            double gmin2, gmax2;
-            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
-            const bool act = pcore && gmin2 <= gap2_max;
-            if (!__any(act)) return false;
-            bool hit = false;
-            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
-            };
-            auto stop = [&] { return static_cast<bool>(__any(hit)); };
-            if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
-            if (lane_id() == 0) uf_unite(bparent, b, b2);
-            return true;
-        });
+    const int32_t nitems = static_cast<int32_t>(nitems_c.get());
+    const int32_t stride = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
+    // waves walk the items wid, wid + stride, ... (one each when the grid
+    // covers them)
+    for (int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                               static_cast<int32_t>(threadIdx.x >> 6));
+         wid < nitems; wid += stride) [&] {
+        // roots of the forest, each searched from its first chunk
+        const int32_t b = uniform(blist[wid]);
+        const int2 it = make_int2(b, uniform(bstart[b]));
+        const int32_t end = uniform(bstart[b + 1]);
+        const int32_t p = it.y + lane_id();
+        const bool pcore = p < end && core[p];
+        if (!__any(pcore)) return;
+        const int32_t u = uniform(bcell[b]);
+        const JobParams<T>& jp = jobs[uniform(job_of_cell[u])];
+        const T eps2 = jp.eps2;
+        const double gap2_max = jp.gap2_max;
+        T pc[D];
+        load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+        const int32_t r0 = uniform(static_cast<int32_t>(roff[u]));
+        const int32_t r1 = uniform(static_cast<int32_t>(roff[u + 1]));
+        for_each_row_box<T, D>(
+            r0, r1, uniform(rcenter[u]), rcells, cbox, bstart, bbox, rsnap,
+            [&](int32_t b2, const double* lo, const double* hi, int32_t q0, int32_t q1,
+                int32_t s2) {
+                if (s2 < 0 || s2 == b) return false;  // no cores, or b's own tree
+                double gmin2, gmax2;
+                point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+                const bool act = pcore && gmin2 <= gap2_max;
+                if (!__any(act)) return false;
+                bool hit = false;
+                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                    if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+                };
+                auto stop = [&] { return static_cast<bool>(__any(hit)); };
+                if (!visit_range<T, D>(soa, ld, core, nullptr, q0, q1, body, stop)) return false;
+                if (lane_id() == 0) uf_unite(bparent, b, b2);
+                return true;
+            });
+    }();
 }
 
 // Box pairs the hook forest leaves open: one thread per box b finds every
@@ -387,11 +400,16 @@ __global__ __launch_bounds__(kThreads) void k_box_root_join(
 // fill pass: the b2 go to list[off[b] ..].
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_scan(
-    int32_t nbox, const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
+    Count nbox_c, const int32_t* __restrict__ bcell, const int32_t* __restrict__ cbox,
     const uint32_t* __restrict__ roff, const int2* __restrict__ rcells,
     const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ job_of_cell,
     const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ rsnap,
-    uint32_t* __restrict__ cnt, int32_t* __restrict__ list, unsigned long long* __restrict__ total) {
+    uint32_t* __restrict__ cnt, int32_t* __restrict__ list, unsigned long long* __restrict__ total,
+    const unsigned long long* __restrict__ total_chk, unsigned long long cap) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
+    // fill pass of the static pipeline: nothing to write when the pairs
+    // exceed the list (the host path then takes over)
+    if (list && total_chk && *total_chk > cap) return;
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     uint32_t k = (list && b < nbox) ? cnt[b] : 0u;
     // the fill pass walks only the boxes the count pass found pairs for
@@ -439,71 +457,78 @@ __global__ __launch_bounds__(kThreads) void k_box_scan(
 // within-eps pair, which joins the boxes.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_union_list(
-    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, Count nitems_c,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const OrdKey<T>* __restrict__ bbox, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
-    const uint32_t* __restrict__ off, const int32_t* __restrict__ list, int32_t* bparent) {
-    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                                static_cast<int32_t>(threadIdx.x >> 6));
-    if (wid >= nitems) return;
-    const int2 it = bitems[wid];
-    const int32_t b = uniform(it.x);
-    const uint32_t l0 = uniform(static_cast<int32_t>(off[b]));
-    const uint32_t l1 = uniform(static_cast<int32_t>(off[b + 1]));
-    if (l0 == l1) return;
-    const int32_t end = uniform(bstart[b + 1]);
-    const int32_t p = it.y + lane_id();
-    const bool pcore = p < end && core[p];
-    if (!__any(pcore)) return;
-    const JobParams<T>& jp = jobs[uniform(job_of_cell[uniform(bcell[b])])];
-    const T eps2 = jp.eps2;
-    const double gap2_max = jp.gap2_max;
-    T pc[D];
-    load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
-    int32_t rb = uniform(uf_find(bparent, b));
-    for (uint32_t L0 = l0; L0 < l1; L0 += kWave) {
-        // lanes fetch 64 partners at once: box, bounding box, rows, and a
-        // root (roots only merge, so an equal root is final)
-        const uint32_t li = min(L0 + static_cast<uint32_t>(lane_id()), l1 - 1);
-        const int32_t b2l = list[li];
-        double lol[D], hil[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) {
-            lol[j] = box_lo<T, D>(bbox, b2l, j);
-            hil[j] = box_hi<T, D>(bbox, b2l, j);
-        }
-        const int32_t q0l = bstart[b2l], q1l = bstart[b2l + 1];
-        const int32_t r2l = uf_find(bparent, b2l);
-        const int32_t nk = static_cast<int32_t>(min(static_cast<uint32_t>(kWave), l1 - L0));
-        for (int32_t j = 0; j < nk; ++j) {
-            double lo[D], hi[D];
-#pragma unroll
-            for (int e = 0; e < D; ++e) {
-                lo[e] = bcast(lol[e], j);
-                hi[e] = bcast(hil[e], j);
+    const uint32_t* __restrict__ off, const int32_t* __restrict__ list, int32_t* bparent,
+    const unsigned long long* __restrict__ total_chk, unsigned long long cap) {
+    if (total_chk && *total_chk > cap) return;  // no list was written
+    const int32_t nitems = static_cast<int32_t>(nitems_c.get());
+    const int32_t stride = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
+    // waves walk the items wid, wid + stride, ... (one each when the grid
+    // covers them)
+    for (int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                               static_cast<int32_t>(threadIdx.x >> 6));
+         wid < nitems; wid += stride) [&] {
+        const int2 it = bitems[wid];
+        const int32_t b = uniform(it.x);
+        const uint32_t l0 = uniform(static_cast<int32_t>(off[b]));
+        const uint32_t l1 = uniform(static_cast<int32_t>(off[b + 1]));
+        if (l0 == l1) return;
+        const int32_t end = uniform(bstart[b + 1]);
+        const int32_t p = it.y + lane_id();
+        const bool pcore = p < end && core[p];
+        if (!__any(pcore)) return;
+        const JobParams<T>& jp = jobs[uniform(job_of_cell[uniform(bcell[b])])];
+        const T eps2 = jp.eps2;
+        const double gap2_max = jp.gap2_max;
+        T pc[D];
+        load_row<T, D>(soa, ld, pcore ? p : it.y, pc);
+        int32_t rb = uniform(uf_find(bparent, b));
+        for (uint32_t L0 = l0; L0 < l1; L0 += kWave) {
+            // lanes fetch 64 partners at once: box, bounding box, rows, and a
+            // root (roots only merge, so an equal root is final)
+            const uint32_t li = min(L0 + static_cast<uint32_t>(lane_id()), l1 - 1);
+            const int32_t b2l = list[li];
+            double lol[D], hil[D];
+    #pragma unroll
+            for (int j = 0; j < D; ++j) {
+                lol[j] = box_lo<T, D>(bbox, b2l, j);
+                hil[j] = box_hi<T, D>(bbox, b2l, j);
             }
-            double gmin2, gmax2;
-            point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
-            const bool act = pcore && gmin2 <= gap2_max;
-            if (!__any(act)) continue;
-            int32_t rb2 = bcast(r2l, j);
-            if (rb2 == rb) continue;
-            rb = uniform(uf_find(bparent, rb));
-            rb2 = uniform(uf_find(bparent, rb2));
-            if (rb2 == rb) continue;
-            bool hit = false;
-            auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
-                if (cq && act && within<D>(pc, qc, eps2)) hit = true;
-            };
-            auto stop = [&] { return static_cast<bool>(__any(hit)); };
-            if (visit_range<T, D>(soa, ld, core, nullptr, bcast(q0l, j), bcast(q1l, j), body,
-                                  stop)) {
-                if (lane_id() == 0) uf_unite(bparent, rb, rb2);
-                rb = uniform(uf_find(bparent, b));
+            const int32_t q0l = bstart[b2l], q1l = bstart[b2l + 1];
+            const int32_t r2l = uf_find(bparent, b2l);
+            const int32_t nk = static_cast<int32_t>(min(static_cast<uint32_t>(kWave), l1 - L0));
+            for (int32_t j = 0; j < nk; ++j) {
+                double lo[D], hi[D];
+    #pragma unroll
+                for (int e = 0; e < D; ++e) {
+                    lo[e] = bcast(lol[e], j);
+                    hi[e] = bcast(hil[e], j);
+                }
+                double gmin2, gmax2;
+                point_box_lh<T, D>(lo, hi, pc, gmin2, gmax2);
+                const bool act = pcore && gmin2 <= gap2_max;
+                if (!__any(act)) continue;
+                int32_t rb2 = bcast(r2l, j);
+                if (rb2 == rb) continue;
+                rb = uniform(uf_find(bparent, rb));
+                rb2 = uniform(uf_find(bparent, rb2));
+                if (rb2 == rb) continue;
+                bool hit = false;
+                auto body = [&](int32_t, const T* qc, uint8_t cq, int32_t) {
+                    if (cq && act && within<D>(pc, qc, eps2)) hit = true;
+                };
+                auto stop = [&] { return static_cast<bool>(__any(hit)); };
+                if (visit_range<T, D>(soa, ld, core, nullptr, bcast(q0l, j), bcast(q1l, j), body,
+                                      stop)) {
+                    if (lane_id() == 0) uf_unite(bparent, rb, rb2);
+                    rb = uniform(uf_find(bparent, b));
+                }
             }
         }
-    }
+    }();
 }
 
 // Hook of a small box (at most kSmallBox rows), one thread per box: the
@@ -514,13 +539,14 @@ __global__ __launch_bounds__(kThreads) void k_box_union_list(
 
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_box_hook_small(
-    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, int32_t nlist,
+    const T* __restrict__ soa, int64_t ld, const int32_t* __restrict__ blist, Count nlist_c,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const int32_t* __restrict__ rcenter,
     const OrdKey<T>* __restrict__ bbox, const int32_t* __restrict__ bmincore,
     const uint8_t* __restrict__ core, const int32_t* __restrict__ job_of_cell,
     const JobParams<T>* __restrict__ jobs, int32_t* __restrict__ bparent) {
+    const int32_t nlist = static_cast<int32_t>(nlist_c.get());
     const int32_t i = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= nlist) return;
     const int32_t b = blist[i];
@@ -646,10 +672,11 @@ __device__ __forceinline__ void block_select(int64_t n, unsigned int* count, Tak
 
 // list[...] = every box b with (mode 0) cores (v = bmincore != INT_MAX) /
 // (mode 1) v[b] == b (a root); *count += their number
-__global__ __launch_bounds__(kThreads) void k_select_boxes(int32_t nbox, int mode,
+__global__ __launch_bounds__(kThreads) void k_select_boxes(Count nbox_c, int mode,
                                                            const int32_t* __restrict__ v,
                                                            int32_t* __restrict__ list,
                                                            unsigned int* __restrict__ count) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     block_select(
         nbox, count,
         [&](int64_t b) { return mode == 0 ? v[b] != INT_MAX : v[b] == static_cast<int32_t>(b); },
@@ -657,11 +684,12 @@ __global__ __launch_bounds__(kThreads) void k_select_boxes(int32_t nbox, int mod
 }
 
 // the 64-slot chunks of the boxes that have open pairs (off[b+1] > off[b])
-__global__ __launch_bounds__(kThreads) void k_select_items(int32_t nitems,
+__global__ __launch_bounds__(kThreads) void k_select_items(Count nitems_c,
                                                            const int2* __restrict__ items,
                                                            const uint32_t* __restrict__ off,
                                                            int2* __restrict__ out,
                                                            unsigned int* __restrict__ count) {
+    const int32_t nitems = static_cast<int32_t>(nitems_c.get());
     block_select(
         nitems, count,
         [&](int64_t i) {
@@ -688,9 +716,10 @@ __global__ __launch_bounds__(kThreads) void k_core_full(int64_t n,
 
 // the chunks holding a row of a box that is not tight with >= minPts rows
 __global__ __launch_bounds__(kThreads) void k_select_core_chunks(
-    int32_t nitems, const int2* __restrict__ items, const int32_t* __restrict__ ustart,
+    Count nitems_c, const int2* __restrict__ items, const int32_t* __restrict__ ustart,
     const int32_t* __restrict__ box_of_slot, const uint8_t* __restrict__ btight,
     int2* __restrict__ out, unsigned int* __restrict__ count) {
+    const int32_t nitems = static_cast<int32_t>(nitems_c.get());
     block_select(
         nitems, count,
         [&](int64_t i) {
@@ -705,10 +734,11 @@ __global__ __launch_bounds__(kThreads) void k_select_core_chunks(
 }
 
 // root of every box's hook tree -> rsnap (-1 for boxes without cores)
-__global__ __launch_bounds__(kThreads) void k_box_roots(int32_t nbox,
+__global__ __launch_bounds__(kThreads) void k_box_roots(Count nbox_c,
                                                         const int32_t* __restrict__ bmincore,
                                                         int32_t* bparent,
                                                         int32_t* __restrict__ rsnap) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox) return;
     rsnap[b] = bmincore[b] == INT_MAX ? -1 : uf_find(bparent, b);
@@ -802,11 +832,12 @@ __global__ __launch_bounds__(kThreads) void k_box_union(
 // the input row); the position itself is the cell's first slot plus the
 // number of the cell's rows with a smaller input row (every box of the
 // cell holds its rows in input order, so a binary search per box counts).
-__global__ __launch_bounds__(kThreads) void k_box_compmin(int32_t nbox,
+__global__ __launch_bounds__(kThreads) void k_box_compmin(Count nbox_c,
                                                           const int32_t* __restrict__ bmincore,
                                                           const int32_t* __restrict__ bcell,
                                                           int32_t* bparent,
                                                           unsigned long long* __restrict__ cmin) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     const bool act = b < nbox && bmincore[b] != INT_MAX;
     // neighbouring boxes mostly share their root: lanes of a run of equal
@@ -835,10 +866,11 @@ __device__ __forceinline__ int32_t count_below(const uint32_t* __restrict__ v, i
 // position of every component root's smallest core, one thread per box
 // (non-roots leave at once)
 __global__ __launch_bounds__(kThreads) void k_box_resolve(
-    int32_t nbox, const int32_t* __restrict__ bparent, const unsigned long long* __restrict__ cmin,
+    Count nbox_c, const int32_t* __restrict__ bparent, const unsigned long long* __restrict__ cmin,
     const int32_t* __restrict__ cbox, const int32_t* __restrict__ bstart,
     const uint32_t* __restrict__ inrow, const int32_t* __restrict__ ustart,
     int32_t* __restrict__ pos) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t r = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (r >= nbox || bparent[r] != r) return;
     const unsigned long long key = cmin[r];
@@ -852,19 +884,21 @@ __global__ __launch_bounds__(kThreads) void k_box_resolve(
 }
 
 // cluster id (smallest core position of the component) per box with cores
-__global__ __launch_bounds__(kThreads) void k_box_cluster(int32_t nbox,
+__global__ __launch_bounds__(kThreads) void k_box_cluster(Count nbox_c,
                                                           const int32_t* __restrict__ bmincore,
                                                           int32_t* bparent,
                                                           const int32_t* __restrict__ compmin,
                                                           int32_t* __restrict__ bcluster) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox) return;
     bcluster[b] = bmincore[b] == INT_MAX ? -1 : compmin[uf_find(bparent, b)];
 }
 
 __global__ __launch_bounds__(kThreads) void k_box_items_count(const int32_t* __restrict__ bstart,
-                                                              int32_t nbox,
+                                                              Count nbox_c,
                                                               uint32_t* __restrict__ cnt) {
+    const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (b >= nbox) return;
     cnt[b] = static_cast<uint32_t>((bstart[b + 1] - bstart[b] + kWave - 1) / kWave);
@@ -938,83 +972,91 @@ void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const ui
     HPD_HIP(hipGetLastError());
 }
 
-void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_boxes, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, bcell,
-                       nbox, m, cbox);
-    HPD_HIP(hipGetLastError());
+void launch_cell_boxes(const int32_t* bcell, Count nbox, Count m, int32_t* cbox, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_boxes, dim3(blocks_for(nbox.host > 0 ? nbox.host : 1, kThreads)),
+                       dim3(kThreads), 0, s, bcell, nbox, m, cbox);
 }
 
-void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32_t* scan_tmp,
+void launch_box_items(const int32_t* bstart, Count nbox, uint32_t* cnt, uint32_t* scan_tmp,
                       uint32_t* total_dev, hipStream_t s) {
-    hipLaunchKernelGGL(k_box_items_count, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       bstart, nbox, cnt);
-    exclusive_scan_u32(cnt, cnt, nbox, scan_tmp, total_dev, s);
-    HPD_HIP(hipGetLastError());
+    if (nbox.host > 0)
+        hipLaunchKernelGGL(k_box_items_count, dim3(blocks_for(nbox.host, kThreads)), dim3(kThreads),
+                           0, s, bstart, nbox, cnt);
+    if (nbox.dev) {
+        exclusive_scan_u32_dev(cnt, cnt, nbox, scan_tmp, s);
+        copy_at_count(cnt, nbox.dev, total_dev, s);
+    } else {
+        exclusive_scan_u32(cnt, cnt, nbox.host, scan_tmp, total_dev, s);
+    }
 }
 
-void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off,
-                           int32_t nitems, int2* items, hipStream_t s) {
+void launch_box_items_fill(const int32_t* bstart, Count nbox, const uint32_t* off, Count nitems,
+                           int2* items, hipStream_t s) {
     launch_expand_chunks(off, nbox, bstart, nitems, items, s);
-    HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D>
 void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s) {
     using U = OrdKey<T>;
+    if (a.nbox <= 0) return;
     hipLaunchKernelGGL((k_box_tight<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
-                       s, reinterpret_cast<const U*>(a.bbox), a.nbox, a.bstart, a.bcell,
-                       a.job_of_cell, a.jobs, btight, nloose);
-    HPD_HIP(hipGetLastError());
+                       s, reinterpret_cast<const U*>(a.bbox), Count{a.nbox, a.nbox_dev}, a.bstart,
+                       a.bcell, a.job_of_cell, a.jobs, btight, nloose);
+}
+
+// a grid of one wave per work item of the count or its bound (the kernels'
+// wave loops then run once; waves past a device-written count exit), at
+// most kMaxItemWaves waves, which walk a larger count
+constexpr int64_t kMaxItemWaves = int64_t{1} << 24;
+inline unsigned wave_blocks(Count items) {
+    const int64_t w = items.host < kMaxItemWaves ? items.host : kMaxItemWaves;
+    return blocks_for(w > 0 ? w : 1, kWavesPerBlock);
 }
 
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
-                     int32_t nlist, int32_t* bparent, hipStream_t s) {
+                     Count nlist, int32_t* bparent, hipStream_t s) {
     using U = OrdKey<T>;
-    hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
-                       bparent, static_cast<int64_t>(a.nbox));
-    if (nlist > 0) {
-        hipLaunchKernelGGL((k_box_hook<T, D>), dim3(blocks_for(nlist, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell,
-                           a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
-                           a.job_of_cell, a.jobs, a.rcenter, bparent);
-        hipLaunchKernelGGL((k_box_hook_small<T, D>), dim3(blocks_for(nlist, kThreads)),
+    if (a.nbox > 0)
+        hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
+                           bparent, static_cast<int64_t>(a.nbox));
+    if (nlist.host > 0) {
+        hipLaunchKernelGGL((k_box_hook<T, D>), dim3(wave_blocks(nlist)), dim3(kThreads), 0, s, a.soa,
+                           a.ld, blist, nlist, a.bstart, a.bcell, a.cbox, roff, rcells,
+                           reinterpret_cast<const U*>(a.bbox), bmincore, core, a.job_of_cell, a.jobs,
+                           a.rcenter, bparent);
+        hipLaunchKernelGGL((k_box_hook_small<T, D>), dim3(blocks_for(nlist.host, kThreads)),
                            dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell,
                            a.cbox, roff, rcells, a.rcenter, reinterpret_cast<const U*>(a.bbox),
                            bmincore, core, a.job_of_cell, a.jobs, bparent);
     }
-    HPD_HIP(hipGetLastError());
 }
 
-void launch_box_roots(int32_t nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
+void launch_box_roots(Count nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
                       hipStream_t s) {
-    if (nbox <= 0) return;
-    hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, nbox,
-                       bmincore, bparent, rsnap);
-    HPD_HIP(hipGetLastError());
+    if (nbox.host <= 0) return;
+    hipLaunchKernelGGL(k_box_roots, dim3(blocks_for(nbox.host, kThreads)), dim3(kThreads), 0, s,
+                       nbox, bmincore, bparent, rsnap);
 }
 
 template <typename T, int D>
 void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                           const int32_t* rsnap, const uint8_t* core, const int32_t* blist,
-                          int32_t nlist, int32_t* bparent, hipStream_t s) {
+                          Count nlist, int32_t* bparent, hipStream_t s) {
     using U = OrdKey<T>;
-    if (nlist <= 0) return;
-    hipLaunchKernelGGL((k_box_root_join<T, D>), dim3(blocks_for(nlist, kWavesPerBlock)),
-                       dim3(kThreads), 0, s, a.soa, a.ld, blist, nlist, a.bstart, a.bcell, a.cbox,
-                       roff, rcells, reinterpret_cast<const U*>(a.bbox), rsnap, core,
-                       a.job_of_cell, a.jobs, a.rcenter, bparent);
-    HPD_HIP(hipGetLastError());
+    if (nlist.host <= 0) return;
+    hipLaunchKernelGGL((k_box_root_join<T, D>), dim3(wave_blocks(nlist)), dim3(kThreads), 0, s,
+                       a.soa, a.ld, blist, nlist, a.bstart, a.bcell, a.cbox, roff, rcells,
+                       reinterpret_cast<const U*>(a.bbox), rsnap, core, a.job_of_cell, a.jobs,
+                       a.rcenter, bparent);
 }
 
-void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list,
+void launch_select_boxes(Count nbox, int mode, const int32_t* v, int32_t* list,
                          unsigned int* count, hipStream_t s) {
-    if (nbox <= 0) return;
-    hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox, kSelectSpan)), dim3(kThreads), 0, s,
-                       nbox, mode, v, list, count);
-    HPD_HIP(hipGetLastError());
+    if (nbox.host <= 0) return;
+    hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox.host, kSelectSpan)), dim3(kThreads), 0,
+                       s, nbox, mode, v, list, count);
 }
 
 void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btight,
@@ -1023,24 +1065,21 @@ void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btig
     if (n <= 0) return;
     hipLaunchKernelGGL(k_core_full, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n,
                        box_of_slot, btight, bstart, inrow, core, bmincore);
-    HPD_HIP(hipGetLastError());
 }
 
-void launch_select_core_chunks(int32_t nitems, const int2* items, const int32_t* ustart,
+void launch_select_core_chunks(Count nitems, const int2* items, const int32_t* ustart,
                                const int32_t* box_of_slot, const uint8_t* btight, int2* out,
                                unsigned int* count, hipStream_t s) {
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL(k_select_core_chunks, dim3(blocks_for(nitems, kSelectSpan)), dim3(kThreads),
-                       0, s, nitems, items, ustart, box_of_slot, btight, out, count);
-    HPD_HIP(hipGetLastError());
+    if (nitems.host <= 0) return;
+    hipLaunchKernelGGL(k_select_core_chunks, dim3(blocks_for(nitems.host, kSelectSpan)),
+                       dim3(kThreads), 0, s, nitems, items, ustart, box_of_slot, btight, out, count);
 }
 
-void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
+void launch_select_items(Count nitems, const int2* items, const uint32_t* off, int2* out,
                          unsigned int* count, hipStream_t s) {
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems, kSelectSpan)), dim3(kThreads), 0,
-                       s, nitems, items, off, out, count);
-    HPD_HIP(hipGetLastError());
+    if (nitems.host <= 0) return;
+    hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems.host, kSelectSpan)), dim3(kThreads),
+                       0, s, nitems, items, off, out, count);
 }
 
 template <typename T, int D>
@@ -1052,55 +1091,53 @@ void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
                        a.job_of_cell, a.jobs, rsnap, bparent);
-    HPD_HIP(hipGetLastError());
 }
 
 template <typename T, int D>
 void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* rsnap, uint32_t* cnt, int32_t* list,
-                     unsigned long long* total, hipStream_t s) {
+                     unsigned long long* total, hipStream_t s,
+                     const unsigned long long* total_chk, unsigned long long cap) {
     using U = OrdKey<T>;
+    if (a.nbox <= 0) return;
     hipLaunchKernelGGL((k_box_scan<T, D>), dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0,
-                       s, a.nbox, a.bcell, a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox),
-                       a.job_of_cell, a.jobs, rsnap, cnt, list, total);
-    HPD_HIP(hipGetLastError());
+                       s, Count{a.nbox, a.nbox_dev}, a.bcell, a.cbox, roff, rcells,
+                       reinterpret_cast<const U*>(a.bbox), a.job_of_cell, a.jobs, rsnap, cnt, list,
+                       total, total_chk, cap);
 }
 
 template <typename T, int D>
-void launch_box_union_list(const BoxArgs<T>& a, const int2* items, int32_t nitems,
+void launch_box_union_list(const BoxArgs<T>& a, const int2* items, Count nitems,
                            const uint8_t* core, const uint32_t* off, const int32_t* list,
-                           int32_t* bparent, hipStream_t s) {
+                           int32_t* bparent, hipStream_t s, const unsigned long long* total_chk,
+                           unsigned long long cap) {
     using U = OrdKey<T>;
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(blocks_for(nitems, kWavesPerBlock)),
-                       dim3(kThreads), 0, s, a.soa, a.ld, items, nitems, a.bstart, a.bcell,
+    if (nitems.host <= 0) return;
+    hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(wave_blocks(nitems)), dim3(kThreads), 0, s,
+                       a.soa, a.ld, items, nitems, a.bstart, a.bcell,
                        reinterpret_cast<const U*>(a.bbox), core, a.job_of_cell, a.jobs, off, list,
-                       bparent);
-    HPD_HIP(hipGetLastError());
+                       bparent, total_chk, cap);
 }
 
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, p, v, n);
-    HPD_HIP(hipGetLastError());
 }
 
-void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* bcell,
+void launch_box_clusters(Count nbox, const int32_t* bmincore, const int32_t* bcell,
                          const int32_t* cbox, const int32_t* bstart, const uint32_t* inrow,
                          const int32_t* ustart, int32_t* bparent, unsigned long long* cmin,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s) {
-    if (nbox <= 0) return;
-    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s, cmin,
-                       ~0ull, static_cast<int64_t>(nbox));
-    hipLaunchKernelGGL(k_box_compmin, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       nbox, bmincore, bcell, bparent, cmin);
-    hipLaunchKernelGGL(k_box_resolve, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       nbox, static_cast<const int32_t*>(bparent),
+    if (nbox.host <= 0) return;
+    const dim3 g(blocks_for(nbox.host, kThreads));
+    hipLaunchKernelGGL(k_fill_u64, g, dim3(kThreads), 0, s, cmin, ~0ull, nbox.host);
+    hipLaunchKernelGGL(k_box_compmin, g, dim3(kThreads), 0, s, nbox, bmincore, bcell, bparent, cmin);
+    hipLaunchKernelGGL(k_box_resolve, g, dim3(kThreads), 0, s, nbox,
+                       static_cast<const int32_t*>(bparent),
                        static_cast<const unsigned long long*>(cmin), cbox, bstart, inrow, ustart,
                        compmin);
-    hipLaunchKernelGGL(k_box_cluster, dim3(blocks_for(nbox, kThreads)), dim3(kThreads), 0, s,
-                       nbox, bmincore, bparent, static_cast<const int32_t*>(compmin), bcluster);
-    HPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_box_cluster, g, dim3(kThreads), 0, s, nbox, bmincore, bparent,
+                       static_cast<const int32_t*>(compmin), bcluster);
 }
 
 // Sorted positions of all slots (for the slot-level union and the
@@ -1146,20 +1183,22 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
     template void launch_box_tight<T, D>(const BoxArgs<T>&, uint8_t*, unsigned int*,             \
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
-                                        const int32_t*, const uint8_t*, const int32_t*, int32_t,  \
+                                        const int32_t*, const uint8_t*, const int32_t*, Count,    \
                                         int32_t*, hipStream_t);                                   \
     template void launch_box_root_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,     \
                                              const int32_t*, const uint8_t*, const int32_t*,      \
-                                             int32_t, int32_t*, hipStream_t);                     \
+                                             Count, int32_t*, hipStream_t);                       \
     template void launch_box_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, const uint8_t*, const int32_t*, int32_t*, \
                                         hipStream_t);                                             \
     template void launch_box_scan<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, uint32_t*, int32_t*,                      \
-                                        unsigned long long*, hipStream_t);                        \
-    template void launch_box_union_list<T, D>(const BoxArgs<T>&, const int2*, int32_t,             \
+                                        unsigned long long*, hipStream_t,                         \
+                                        const unsigned long long*, unsigned long long);           \
+    template void launch_box_union_list<T, D>(const BoxArgs<T>&, const int2*, Count,               \
                                               const uint8_t*, const uint32_t*, const int32_t*,    \
-                                              int32_t*, hipStream_t);
+                                              int32_t*, hipStream_t, const unsigned long long*,   \
+                                              unsigned long long);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \
     HPD_INST_BOX_KEYS(T, 2)   \

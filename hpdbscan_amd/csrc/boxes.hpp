@@ -18,6 +18,7 @@ struct BoxArgs {
     const T* soa = nullptr;            // coordinates in slot order, [D][ld]
     int64_t ld = 0;
     int32_t nbox = 0;
+    const uint32_t* nbox_dev = nullptr;  // static pipeline: nbox in device memory (nbox a bound)
     const int32_t* bstart = nullptr;   // first slot of every box, [nbox + 1]
     const int32_t* bcell = nullptr;    // occupied cell of every box
     const int32_t* cbox = nullptr;     // first box of every cell, [m + 1]
@@ -49,14 +50,13 @@ void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const ui
                       uint32_t* flags, uint32_t* scan, uint32_t* scan_tmp, int32_t* bstart,
                       int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev, void* bbox,
                       int bbox_keys, int key_bytes, hipStream_t s);
-void launch_cell_boxes(const int32_t* bcell, int32_t nbox, int32_t m, int32_t* cbox,
-                       hipStream_t s);
+void launch_cell_boxes(const int32_t* bcell, Count nbox, Count m, int32_t* cbox, hipStream_t s);
 
 // cnt[b] = 64-slot chunks of box b, scanned in place; *total_dev = chunks
-void launch_box_items(const int32_t* bstart, int32_t nbox, uint32_t* cnt, uint32_t* scan_tmp,
+void launch_box_items(const int32_t* bstart, Count nbox, uint32_t* cnt, uint32_t* scan_tmp,
                       uint32_t* total_dev, hipStream_t s);
-void launch_box_items_fill(const int32_t* bstart, int32_t nbox, const uint32_t* off,
-                           int32_t nitems, int2* items, hipStream_t s);
+void launch_box_items_fill(const int32_t* bstart, Count nbox, const uint32_t* off, Count nitems,
+                           int2* items, hipStream_t s);
 
 // btight[b]: bit 0 tight, bit 1 tight with >= minPts rows; *nloose counts
 // boxes of two or more rows that are not tight
@@ -77,18 +77,18 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
 template <int D>
 constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
 
-void launch_select_boxes(int32_t nbox, int mode, const int32_t* v, int32_t* list,
+void launch_select_boxes(Count nbox, int mode, const int32_t* v, int32_t* list,
                          unsigned int* count, hipStream_t s);
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
-                     int32_t nlist, int32_t* bparent, hipStream_t s);
-void launch_box_roots(int32_t nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
+                     Count nlist, int32_t* bparent, hipStream_t s);
+void launch_box_roots(Count nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
                       hipStream_t s);
 template <typename T, int D>
 void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                           const int32_t* rsnap, const uint8_t* core, const int32_t* blist,
-                          int32_t nlist, int32_t* bparent, hipStream_t s);
+                          Count nlist, int32_t* bparent, hipStream_t s);
 // the box chunks (items) of the boxes with open pairs (off[b+1] > off[b])
 // core[s] = 1 for the rows of tight boxes with >= minPts rows (btight bit 1),
 // 0 for every other row; for those boxes bmincore[b] = inrow[bstart[b]]
@@ -98,10 +98,10 @@ void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btig
                       int32_t* bmincore, hipStream_t s);
 // out[..] = the chunks (cell, first row) of items holding a row outside
 // those boxes; *count += their number
-void launch_select_core_chunks(int32_t nitems, const int2* items, const int32_t* ustart,
+void launch_select_core_chunks(Count nitems, const int2* items, const int32_t* ustart,
                                const int32_t* box_of_slot, const uint8_t* btight, int2* out,
                                unsigned int* count, hipStream_t s);
-void launch_select_items(int32_t nitems, const int2* items, const uint32_t* off, int2* out,
+void launch_select_items(Count nitems, const int2* items, const uint32_t* off, int2* out,
                          unsigned int* count, hipStream_t s);
 template <typename T, int D>
 void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
@@ -114,18 +114,23 @@ void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
 // list[cnt[b] ..] (cnt scanned to offsets in between).
 // launch_box_union_list joins the listed pairs that hold a within-eps core
 // pair; off has nbox + 1 entries.  When the list does not fit, the caller
-// runs launch_box_join instead.
+// runs launch_box_join instead.  With total_chk (the static pipeline, no
+// host check in between), both do nothing when *total_chk > cap.
 template <typename T, int D>
 void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* rsnap, uint32_t* cnt, int32_t* list,
-                     unsigned long long* total, hipStream_t s);
+                     unsigned long long* total, hipStream_t s,
+                     const unsigned long long* total_chk = nullptr, unsigned long long cap = 0);
 template <typename T, int D>
-void launch_box_union_list(const BoxArgs<T>& a, const int2* items, int32_t nitems,
+void launch_box_union_list(const BoxArgs<T>& a, const int2* items, Count nitems,
                            const uint8_t* core, const uint32_t* off, const int32_t* list,
-                           int32_t* bparent, hipStream_t s);// p[0, n) = v
+                           int32_t* bparent, hipStream_t s,
+                           const unsigned long long* total_chk = nullptr,
+                           unsigned long long cap = 0);
+// p[0, n) = v
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 
-void launch_box_clusters(int32_t nbox, const int32_t* bmincore, const int32_t* bcell,
+void launch_box_clusters(Count nbox, const int32_t* bmincore, const int32_t* bcell,
                          const int32_t* cbox, const int32_t* bstart, const uint32_t* inrow,
                          const int32_t* ustart, int32_t* bparent, unsigned long long* cmin,
                          int32_t* compmin, int32_t* bcluster, hipStream_t s);

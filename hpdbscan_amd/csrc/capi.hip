@@ -228,6 +228,21 @@ struct Context {
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
     std::vector<hipEvent_t> events;
+    // hpd_context_set_graph: floating-point runs go through the static
+    // pipeline, captured in a hipGraph and replayed while the shape stays
+    bool graph = false;
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        int dtype = -1, d = 0, border_rule = -1, sparse_max = -1;
+        bool sparse_core = false;
+        int64_t n = 0;
+        const void* data = nullptr;
+        const void* labels = nullptr;
+        const void* arena_main = nullptr;
+        const void* arena_aux = nullptr;
+        std::vector<int64_t> offsets;
+    } graph_entry;
+    uint64_t graph_launches = 0, graph_captures = 0, graph_fallbacks = 0;
 };
 
 struct DeviceGuard {
@@ -439,14 +454,14 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                       : nullptr;
     // up to 27 rows per cell (D <= 4): the copy puts the nearest rows first
     const bool by_class = stash && D <= 4;
-    launch_neighbor_count<K, T>(ucell, ustart, m, jobs, njobs, job_of_cell, nranges, ccnt,
+    launch_neighbor_count<K, T>(ucell, ustart, host_count(m), jobs, njobs, job_of_cell, nranges, ccnt,
                                 &dstats[0], &dstats[4], stash, stash_rows, rcenter, by_class, s);
-    launch_chunk_counts(ustart, m, nchunks, s);
+    launch_chunk_counts(ustart, host_count(m), nchunks, s);
     // box mode: dense-cell chunks and sparse-cell rows as separate work
     // lists; their per-cell counts live in the (now unused) run flags
     uint32_t* dcnt = flags;
     uint32_t* scnt = flags + nn + 1;
-    if (boxes) launch_split_counts(ustart, m, ctx.sparse_max, dcnt, scnt, s);
+    if (boxes) launch_split_counts(ustart, host_count(m), ctx.sparse_max, dcnt, scnt, s);
     // the per-cell counts, scanned together (totals at entry m)
     ScanSet counts{};
     int ncounts = 0;
@@ -480,7 +495,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     const int32_t nsitems = boxes ? static_cast<int32_t>(hc[7]) : 0;
     int32_t nbitems = 0;
     if (boxes) {
-        launch_box_items(bstart, nbox, bcnt, tmp, &counters[4], s);
+        launch_box_items(bstart, host_count(nbox), bcnt, tmp, &counters[4], s);
         HPD_HIP(hipMemcpyAsync(hc + 4, &counters[4], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HPD_HIP(hipStreamSynchronize(s));
         nbitems = static_cast<int32_t>(hc[4]);
@@ -506,12 +521,13 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         launch_neighbor_copy(stash, stash_rows, ustart, m, nranges, ranges, rcells, rcenter,
                              by_class, s);
     else
-        launch_neighbor_fill<K, T>(ucell, ustart, m, jobs, job_of_cell, nranges, ranges, rcells,
-                                   rcenter, s);
-    launch_work_items(ustart, m, nchunks, nitems, items, s);
+        launch_neighbor_fill<K, T>(ucell, ustart, host_count(m), jobs, job_of_cell, nranges, ranges,
+                                   rcells, rcenter, s);
+    launch_work_items(ustart, host_count(m), nchunks, host_count(nitems), items, s);
     if (boxes) {
-        launch_split_items(ustart, m, ctx.sparse_max, dcnt, scnt, ditems, nditems, sitems, s);
-        launch_expand_chunks(ccnt, m, ustart, ncitems, citems, s);
+        launch_split_items(ustart, host_count(m), ctx.sparse_max, dcnt, scnt, ditems,
+                           host_count(nditems), sitems, s);
+        launch_expand_chunks(ccnt, host_count(m), ustart, host_count(ncitems), citems, s);
     }
 
     DbscanArgs<T> a;
@@ -535,8 +551,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     BoxArgs<T> ba;
     bool box_union = false;
     if (boxes) {
-        launch_cell_boxes(bcell, nbox, m, cbox, s);
-        launch_box_items_fill(bstart, nbox, bcnt, nbitems, bitems, s);
+        launch_cell_boxes(bcell, host_count(nbox), host_count(m), cbox, s);
+        launch_box_items_fill(bstart, host_count(nbox), bcnt, host_count(nbitems), bitems, s);
         ba.soa = soa;
         ba.ld = ld;
         ba.nbox = nbox;
@@ -577,7 +593,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     int32_t* blist = parent;
     int32_t nlist = 0;
     if (boxes) {
-        launch_select_boxes(nbox, 0, bmincore, blist, &counters[8], s);
+        launch_select_boxes(host_count(nbox), 0, bmincore, blist, &counters[8], s);
         HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HPD_HIP(hipStreamSynchronize(s));
         box_union = hc[5] == 0;  // every box of two or more rows is tight
@@ -586,16 +602,17 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     if (box_union) {
         // compmin holds the forest's roots until the clusters are known
         tm.mark(M_HOOK0);
-        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, nlist, bparent, s);
-        launch_box_roots(nbox, bmincore, bparent, compmin, s);
+        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, host_count(nlist),
+                              bparent, s);
+        launch_box_roots(host_count(nbox), bmincore, bparent, compmin, s);
         for (int round = 0; round < kRootJoinRounds<D>; ++round) {
             HPD_HIP(hipMemsetAsync(&counters[9], 0, sizeof(uint32_t), s));
-            launch_select_boxes(nbox, 1, compmin, blist, &counters[9], s);
+            launch_select_boxes(host_count(nbox), 1, compmin, blist, &counters[9], s);
             HPD_HIP(hipMemcpyAsync(hc + 9, &counters[9], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HPD_HIP(hipStreamSynchronize(s));
             launch_box_root_join<T, D>(ba, nranges, rcells, compmin, core, blist,
-                                       static_cast<int32_t>(hc[9]), bparent, s);
-            launch_box_roots(nbox, bmincore, bparent, compmin, s);
+                                       host_count(hc[9]), bparent, s);
+            launch_box_roots(host_count(nbox), bmincore, bparent, compmin, s);
         }
         tm.mark(M_HOOK1);
         // the open pairs of every box, as a list in scratch64; offsets in
@@ -606,7 +623,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         // offsets (meaningful when the total fits, checked below) and the
         // chunks of the boxes that have open pairs
         exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
-        launch_select_items(nbitems, bitems, bcnt, uitems, &counters[10], s);
+        launch_select_items(host_count(nbitems), bitems, bcnt, uitems, &counters[10], s);
         unsigned long long* hs64 = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
         HPD_HIP(hipMemcpyAsync(hs64 + 8, &dstats[5], sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, s));
@@ -615,14 +632,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         ctx.stats.open_box_pairs = hs64[8];
         if (hs64[8] <= cap) {  // then the 32-bit offsets cannot overflow
             launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s);
-            launch_box_union_list<T, D>(ba, uitems, static_cast<int32_t>(hc[20]), core, bcnt, list,
-                                        bparent, s);
+            launch_box_union_list<T, D>(ba, uitems, host_count(hc[20]), core, bcnt, list, bparent,
+                                        s);
         } else {
             launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
         }
         tm.mark(M_JOIN1);
-        launch_box_clusters(nbox, bmincore, bcell, cbox, bstart, inrow_of_slot, ustart, bparent,
-                            reinterpret_cast<unsigned long long*>(scratch64), compmin, bcluster, s);
+        launch_box_clusters(host_count(nbox), bmincore, bcell, cbox, bstart, inrow_of_slot, ustart,
+                            bparent, reinterpret_cast<unsigned long long*>(scratch64), compmin,
+                            bcluster, s);
     } else {
         if (boxes) {
             // a loose box: the slot-level union needs the sorted positions
@@ -690,12 +708,414 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     ctx.stats.box_union = box_union ? 1 : 0;
 }
 
+// --------------------------------------------------------------------------
+// the static pipeline (hpd_context_set_graph)
+// --------------------------------------------------------------------------
+// The same kernels as run_keyed, with every table sized by its worst case
+// from n, the number of jobs and D, and every count that run_keyed reads back
+// to size the next step left in device memory (a Count with a device
+// address, the kernels reading it; wave kernels walk their items from a
+// bounded grid).  The grid of every job is computed on the device
+// (launch_geometry) and the radix passes beyond its key bits do nothing, so
+// no host read-back happens between the first kernel and the last: the
+// sequence can be captured in a hipGraph and replayed.  Conditions on which
+// run_keyed branches become flags read after the run; any of them hands the
+// run to the plain pipeline: non-finite input, grids beyond 64 key bits,
+// loose boxes, more open box pairs than the list holds.  Box mode only
+// (floating-point data), smallest-cluster border rule.
+
+// Device words the static pipeline keeps its counts in.
+enum StaticSlot : int {
+    SS_M = 0,          // occupied cells
+    SS_RANGES = 1,     // neighbour rows
+    SS_ITEMS = 2,      // 64-row chunks of every cell
+    SS_NBOX = 3,       // boxes
+    SS_BITEMS = 4,     // 64-slot chunks of every box
+    SS_LOOSE = 5,      // boxes of two or more rows that are not tight
+    SS_DITEMS = 6,     // chunks of dense cells
+    SS_SITEMS = 7,     // rows of sparse cells
+    SS_NLIST = 8,      // boxes with cores
+    SS_JOIN0 = 9,      // roots of the first root-join round
+    SS_UITEMS = 10,    // box chunks with open pairs
+    SS_CITEMS = 11,    // chunks of cells with >= minPts candidates
+    SS_NEED = 12,      // those holding a row outside the full tight boxes
+    SS_JOIN1 = 13,     // roots of the second round
+    SS_NONFINITE = 14, // non-finite coordinates
+    SS_GRID = 15,      // grid beyond 64 key bits
+    SS_BITS = 16,      // key bits + sub-box bits
+    SS_WORDS = 32
+};
+
+// worst-case sizes of the static pipeline's work lists
+static inline int64_t item_bound(int64_t n) { return n + n / 64 + 16; }  // chunks: <= m + n / 64
+template <int D>
+static inline int64_t range_bound(int64_t n) {
+    int64_t r = n;
+    for (int j = 1; j < D; ++j) r *= 3;
+    return r;
+}
+
+template <typename T, int D>
+static size_t static_aux_bytes(int64_t n) {
+    return 2 * align_up(static_cast<size_t>(range_bound<D>(n)) * sizeof(int2), 256) +
+           7 * align_up(static_cast<size_t>(item_bound(n)) * sizeof(int2), 256) + 4096;
+}
+
+template <typename T, int D>
+static size_t static_main_bytes(int64_t n, int32_t njobs) {
+    return workspace_bytes<T, D>(n, njobs, false) + sizeof(uint64_t) * static_cast<size_t>(njobs) +
+           SS_WORDS * 4 + 64 * 1024 + 8 * 256;
+}
+
+// The body of a static run on stream s: every launch of the run, nothing
+// else (no read-back, no synchronisation, no memset), in the workspace the
+// caller reserved and started taking from; jobs / bfirst_dev hold the
+// host-made job table (row ranges, eps2, minPts, reciprocals) and min/max
+// blocks.  sc (SS_WORDS words) and dstats (16 + kNoiseStripes words) end up
+// holding the counts read afterwards.
+template <typename T, int D>
+static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* jobs,
+                        const int32_t* bfirst_dev, int32_t njobs, int32_t nblocks, int64_t* labels,
+                        uint32_t* sc, unsigned long long* dstats, hipStream_t s) {
+    using K = uint64_t;
+    Arena& A = ctx.main;
+    Arena& X = ctx.aux;
+    X.reset();
+    const size_t nn = static_cast<size_t>(n);
+    const int64_t ld = (n + 15) & ~int64_t(15);
+    const size_t L = static_cast<size_t>(ld);
+    const int64_t IB = item_bound(n);
+    // zeroed by a kernel (no memset in a graph)
+    fill_i32(reinterpret_cast<int32_t*>(sc), 0, SS_WORDS, s);
+    fill_i32(reinterpret_cast<int32_t*>(dstats), 0, 2 * (16 + kNoiseStripes), s);
+    unsigned long long* noise_stripes = dstats + 16;
+
+    // min / max, then every job's grid on the device
+    T* part = A.take_n<T>(static_cast<size_t>(nblocks) * 2 * D + 1);
+    T* mm = A.take_n<T>(static_cast<size_t>(njobs) * 2 * D + 16);
+    uint64_t* jcells = A.take_n<uint64_t>(static_cast<size_t>(njobs));
+    unsigned long long* total_cells = dstats + 6;
+    launch_minmax<T, D>(pts, jobs, njobs, bfirst_dev, nblocks, part, mm,
+                        reinterpret_cast<int*>(&sc[SS_NONFINITE]), s);
+    const int sub_bits = box_sub_bits(D);
+    launch_geometry<T, D>(jobs, njobs, mm, sub_bits, jcells, reinterpret_cast<int*>(&sc[SS_GRID]),
+                          reinterpret_cast<int*>(&sc[SS_BITS]), total_cells, s);
+
+    K* keys_a = A.take_n<K>(nn);
+    K* keys_b = A.take_n<K>(nn);
+    uint32_t* vals_a = A.take_n<uint32_t>(nn);
+    uint32_t* vals_b = A.take_n<uint32_t>(nn);
+    const size_t hist_words = radix_hist_words(n);
+    uint32_t* hist = A.take_n<uint32_t>(hist_words);
+    const size_t tmp_words = scan_tmp_words(static_cast<int64_t>(std::max(hist_words, 2 * nn + 2)));
+    uint32_t* tmp = A.take_n<uint32_t>(tmp_words);
+    T* soa = A.take_n<T>(L * D);
+    uint32_t* flags = A.take_n<uint32_t>(2 * nn + 2);
+    K* ucell = A.take_n<K>(nn);
+    int32_t* ustart = A.take_n<int32_t>(nn + 1);
+    uint32_t* nranges = A.take_n<uint32_t>(nn + 1);
+    uint32_t* nchunks = A.take_n<uint32_t>(nn + 1);
+    uint8_t* core = A.take_n<uint8_t>(L);
+    int32_t* parent = A.take_n<int32_t>(L);
+    (void)A.take_n<int32_t>(L);  // rsnap of the slot-level union (not used here)
+    int32_t* rootslot = A.take_n<int32_t>(L);
+    int32_t* cmin = A.take_n<int32_t>(nn);
+    int32_t* cmax = A.take_n<int32_t>(nn);
+    int32_t* rcenter = A.take_n<int32_t>(nn);
+    int32_t* job_of_cell = A.take_n<int32_t>(nn);
+    uint64_t* scratch64 = A.take_n<uint64_t>(nn);
+    (void)A.take_n<uint32_t>(nn);  // sorted positions (loose boxes only)
+    int32_t* box_of_slot = A.take_n<int32_t>(nn);
+    int32_t* bstart = A.take_n<int32_t>(nn + 1);
+    int32_t* bcell = A.take_n<int32_t>(nn);
+    int32_t* cbox = A.take_n<int32_t>(nn + 1);
+    uint32_t* bcnt = A.take_n<uint32_t>(nn + 1);
+    int32_t* bmincore = A.take_n<int32_t>(nn);
+    int32_t* bparent = A.take_n<int32_t>(nn);
+    int32_t* compmin = A.take_n<int32_t>(nn);
+    int32_t* bcluster = A.take_n<int32_t>(nn);
+    uint8_t* btight = A.take_n<uint8_t>(nn);
+    void* bbox = A.take(2 * D * sizeof(T) * nn);
+    void* bpart = A.take(gather_part_bytes(n, D, static_cast<int>(sizeof(T))));
+    uint32_t* ccnt = A.take_n<uint32_t>(nn + 1);
+
+    // cell ids and the sort (passes past the key's bits do nothing)
+    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, box_split(D), sub_bits, keys_a, vals_a, s);
+    radix_sort_pairs_dev<K>(keys_a, vals_a, keys_b, vals_b, n, 64,
+                            reinterpret_cast<const int*>(&sc[SS_BITS]), hist, tmp, s);
+    K* keys = keys_a;
+    uint32_t* inrow_of_slot = vals_a;
+    launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &sc[SS_M], s);
+    launch_box_table<K>(keys, n, flags, flags + n, vals_b, reinterpret_cast<uint32_t*>(keys_b), tmp,
+                        bstart, bcell, box_of_slot, &sc[SS_NBOX], nullptr, 2 * D,
+                        static_cast<int>(sizeof(T)), s);
+    launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, bpart, s);
+
+    // neighbour rows and work lists, counted and laid out on the device
+    const Count M{n, &sc[SS_M]};
+    const Count NB{n, &sc[SS_NBOX]};
+    uint32_t* dcnt = flags;
+    uint32_t* scnt = flags + nn + 1;
+    launch_neighbor_count<K, T>(ucell, ustart, M, jobs, njobs, job_of_cell, nranges, ccnt,
+                                &dstats[0], &dstats[4], nullptr, 1, rcenter, false, s);
+    launch_chunk_counts(ustart, M, nchunks, s);
+    launch_split_counts(ustart, M, ctx.sparse_max, dcnt, scnt, s);
+    ScanSet counts{};
+    counts.a[0] = nranges;
+    counts.a[1] = nchunks;
+    counts.a[2] = ccnt;
+    counts.a[3] = dcnt;
+    counts.a[4] = scnt;
+    exclusive_scan_set_u32(counts, 5, n, tmp, tmp_words, s, &sc[SS_M]);
+    copy_at_count(nranges, &sc[SS_M], &sc[SS_RANGES], s);
+    copy_at_count(nchunks, &sc[SS_M], &sc[SS_ITEMS], s);
+    copy_at_count(ccnt, &sc[SS_M], &sc[SS_CITEMS], s);
+    copy_at_count(dcnt, &sc[SS_M], &sc[SS_DITEMS], s);
+    copy_at_count(scnt, &sc[SS_M], &sc[SS_SITEMS], s);
+    launch_box_items(bstart, NB, bcnt, tmp, &sc[SS_BITEMS], s);
+
+    const int64_t R = range_bound<D>(n);
+    int2* ranges = X.take_n<int2>(static_cast<size_t>(R));
+    int2* rcells = X.take_n<int2>(static_cast<size_t>(R));
+    int2* items = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* bitems = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* ditems = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* sitems = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* uitems = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* citems = X.take_n<int2>(static_cast<size_t>(IB));
+    int2* citems_need = X.take_n<int2>(static_cast<size_t>(IB));
+    launch_neighbor_fill<K, T>(ucell, ustart, M, jobs, job_of_cell, nranges, ranges, rcells, rcenter,
+                               s);
+    launch_work_items(ustart, M, nchunks, Count{IB, &sc[SS_ITEMS]}, items, s);
+    launch_split_items(ustart, M, ctx.sparse_max, dcnt, scnt, ditems, Count{IB, &sc[SS_DITEMS]},
+                       sitems, s);
+    launch_expand_chunks(ccnt, M, ustart, Count{IB, &sc[SS_CITEMS]}, citems, s);
+    launch_cell_boxes(bcell, NB, M, cbox, s);
+    launch_box_items_fill(bstart, NB, bcnt, Count{IB, &sc[SS_BITEMS]}, bitems, s);
+
+    BoxArgs<T> ba;
+    ba.soa = soa;
+    ba.ld = ld;
+    ba.nbox = static_cast<int32_t>(n);
+    ba.nbox_dev = &sc[SS_NBOX];
+    ba.bstart = bstart;
+    ba.bcell = bcell;
+    ba.cbox = cbox;
+    ba.bitems = bitems;
+    ba.nbitems = static_cast<int32_t>(IB);
+    ba.bbox = bbox;
+    ba.jobs = jobs;
+    ba.job_of_cell = job_of_cell;
+    ba.rcenter = rcenter;
+    launch_box_tight<T, D>(ba, btight, &sc[SS_LOOSE], s);
+    fill_i32(bmincore, INT_MAX, n, s);
+
+    DbscanArgs<T> a;
+    a.soa = soa;
+    a.n = n;
+    a.ld = ld;
+    a.items = items;
+    a.nitems = static_cast<int32_t>(IB);
+    a.ustart = ustart;
+    a.roff = nranges;
+    a.ranges = ranges;
+    a.rcells = rcells;
+    a.rcenter = rcenter;
+    a.m = static_cast<int32_t>(n);
+    a.jobs = jobs;
+    a.njobs = njobs;
+    a.job_of_cell = job_of_cell;
+    a.pos_of_slot = nullptr;
+    a.inrow = inrow_of_slot;
+    a.box_of_slot = box_of_slot;
+    a.btight = btight;
+    a.cbox = cbox;
+    a.bstart = bstart;
+    a.bcell = bcell;
+    a.bbox = bbox;
+    a.sparse_core = ctx.sparse_core;
+    a.bmincore = bmincore;
+    a.ditems = ditems;
+    a.nditems = static_cast<int32_t>(IB);
+    a.nditems_dev = &sc[SS_DITEMS];
+    a.citems = citems;
+    a.ncitems = static_cast<int32_t>(IB);
+    a.ncitems_dev = &sc[SS_CITEMS];
+    a.citems_need = citems_need;
+    a.ncitems_need_dev = &sc[SS_NEED];
+    a.sitems = sitems;
+    a.nsitems = static_cast<int32_t>(n);
+    a.nsitems_dev = &sc[SS_SITEMS];
+    launch_core<T, D>(a, core, &dstats[1], s);
+
+    // the box union (its result is discarded when a box is loose)
+    int32_t* blist = parent;
+    launch_select_boxes(NB, 0, bmincore, blist, &sc[SS_NLIST], s);
+    launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, Count{n, &sc[SS_NLIST]},
+                          bparent, s);
+    launch_box_roots(NB, bmincore, bparent, compmin, s);
+    for (int round = 0; round < kRootJoinRounds<D>; ++round) {
+        uint32_t* cnt = &sc[round == 0 ? SS_JOIN0 : SS_JOIN1];
+        launch_select_boxes(NB, 1, compmin, blist, cnt, s);
+        launch_box_root_join<T, D>(ba, nranges, rcells, compmin, core, blist, Count{n, cnt}, bparent,
+                                   s);
+        launch_box_roots(NB, bmincore, bparent, compmin, s);
+    }
+    int32_t* list = reinterpret_cast<int32_t*>(scratch64);
+    const unsigned long long cap = 2 * static_cast<unsigned long long>(nn);
+    launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
+    exclusive_scan_u32_dev(bcnt, bcnt, NB, tmp, s);
+    launch_select_items(Count{IB, &sc[SS_BITEMS]}, bitems, bcnt, uitems, &sc[SS_UITEMS], s);
+    launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s, &dstats[5], cap);
+    launch_box_union_list<T, D>(ba, uitems, Count{IB, &sc[SS_UITEMS]}, core, bcnt, list, bparent, s,
+                                &dstats[5], cap);
+    launch_box_clusters(NB, bmincore, bcell, cbox, bstart, inrow_of_slot, ustart, bparent,
+                        reinterpret_cast<unsigned long long*>(scratch64), compmin, bcluster, s);
+
+    launch_label_cores<T>(a, core, bcluster, parent, compmin, nullptr, cmin, cmax, labels,
+                          &dstats[1], s);
+    launch_border<T, D>(a, core, bcluster, rootslot, cmin, cmax, labels, &dstats[2], nullptr,
+                        noise_stripes, s);
+}
+
+// Ends a stream capture if the body throws halfway.
+struct CaptureGuard {
+    hipStream_t s;
+    bool open = true;
+    ~CaptureGuard() {
+        if (open) {
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        }
+    }
+};
+
+// One batch through the static pipeline: replayed from the context's graph
+// when the shape (dtype, D, rows, job offsets, buffers, options, workspace)
+// matches the captured one, captured first otherwise.  Returns false when a
+// flag hands the batch to the plain pipeline.
+template <typename T, int D>
+static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const int64_t* offsets,
+                       int32_t njobs, const float* eps, const uint64_t* min_pts, int64_t* labels,
+                       hipStream_t s) {
+    if (range_bound<D>(n) > 0x7fffffffll) return false;  // 32-bit range offsets
+    ctx.main.reserve(static_main_bytes<T, D>(n, njobs));
+    ctx.aux.reserve(static_aux_bytes<T, D>(n));
+    Arena& A = ctx.main;
+    A.reset();
+    JobParams<T>* jobs_dev = A.take_n<JobParams<T>>(static_cast<size_t>(njobs));
+    int32_t* bfirst_dev = A.take_n<int32_t>(static_cast<size_t>(njobs) + 1);
+    uint32_t* sc = A.take_n<uint32_t>(SS_WORDS);
+    unsigned long long* dstats = A.take_n<unsigned long long>(16 + kNoiseStripes);
+
+    // host-made part of the job table: row ranges, eps2, minPts, reciprocals
+    HostArray<JobParams<T>, 1> hj(static_cast<size_t>(njobs));
+    HostArray<int32_t, 2> bfirst(static_cast<size_t>(njobs) + 1);
+    const int rpb = minmax_rows_per_block();
+    int32_t nblocks = 0;
+    for (int32_t j = 0; j < njobs; ++j) {
+        JobParams<T>& jp = hj[j];
+        jp = JobParams<T>{};
+        jp.row_begin = offsets[j];
+        jp.row_end = offsets[j + 1];
+        jp.min_pts = static_cast<uint32_t>(std::min<uint64_t>(min_pts[j], 0xffffffffull));
+        jp.eps2 = eps2_as<T>(eps[j]);
+        jp.gap2_max = static_cast<double>(jp.eps2) * (1.0 + 1e-5) + 1e-30;
+        jp.rcp_f = flushf(1.0f / flushf(eps[j]));
+        jp.rcp_d = flushd(1.0 / flushd(static_cast<double>(eps[j])));
+        jp.cp.d = D;
+        bfirst[j] = nblocks;
+        nblocks += static_cast<int32_t>((offsets[j + 1] - offsets[j] + rpb - 1) / rpb);
+    }
+    bfirst[njobs] = nblocks;
+    HPD_HIP(hipMemcpyAsync(jobs_dev, hj.data(), sizeof(JobParams<T>) * njobs, hipMemcpyHostToDevice,
+                           s));
+    HPD_HIP(hipMemcpyAsync(bfirst_dev, bfirst.data(), sizeof(int32_t) * (njobs + 1),
+                           hipMemcpyHostToDevice, s));
+
+    Context::GraphEntry& ge = ctx.graph_entry;
+    const bool same = ge.exec && ge.dtype == dtype && ge.d == D && ge.n == n && ge.data == pts &&
+                      ge.labels == labels && ge.arena_main == A.base() &&
+                      ge.arena_aux == ctx.aux.base() && ge.border_rule == ctx.border_rule &&
+                      ge.sparse_max == ctx.sparse_max && ge.sparse_core == ctx.sparse_core &&
+                      ge.offsets.size() == static_cast<size_t>(njobs) + 1 &&
+                      std::equal(ge.offsets.begin(), ge.offsets.end(), offsets);
+    if (!same) {
+        if (ge.exec) {
+            (void)hipGraphExecDestroy(ge.exec);
+            ge.exec = nullptr;
+        }
+        // captured on the context's stream, launched on s
+        HPD_HIP(hipStreamSynchronize(ctx.stream));
+        HPD_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+        CaptureGuard guard{ctx.stream};
+        static_body<T, D>(ctx, pts, n, jobs_dev, bfirst_dev, njobs, nblocks, labels, sc, dstats,
+                          ctx.stream);
+        hipGraph_t g = nullptr;
+        guard.open = false;
+        HPD_HIP(hipStreamEndCapture(ctx.stream, &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HPD_HIP(e);
+        ge.exec = exec;
+        ge.dtype = dtype;
+        ge.d = D;
+        ge.n = n;
+        ge.data = pts;
+        ge.labels = labels;
+        ge.arena_main = A.base();
+        ge.arena_aux = ctx.aux.base();
+        ge.border_rule = ctx.border_rule;
+        ge.sparse_max = ctx.sparse_max;
+        ge.sparse_core = ctx.sparse_core;
+        ge.offsets.assign(offsets, offsets + njobs + 1);
+        ++ctx.graph_captures;
+    }
+    HPD_HIP(hipGraphLaunch(ge.exec, s));
+    ++ctx.graph_launches;
+    // one read-back after the run: counts, flags and statistics
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx.host_scratch);
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch) + 16;
+    HPD_HIP(hipMemcpyAsync(hc, sc, SS_WORDS * 4, hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hs, dstats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipStreamSynchronize(s));
+    const bool ok = hc[SS_NONFINITE] == 0 && hc[SS_GRID] == 0 && hc[SS_LOOSE] == 0 &&
+                    hs[5] <= 2 * static_cast<unsigned long long>(n) &&
+                    hs[4] <= static_cast<unsigned long long>(range_bound<D>(n));
+    if (!ok) {
+        ++ctx.graph_fallbacks;
+        return false;
+    }
+    ctx.stats.sum_candidates = hs[0];
+    ctx.stats.core_points = hs[1];
+    ctx.stats.noise_points = hs[2];
+    ctx.stats.ambiguous_borders = hs[3];
+    ctx.stats.open_box_pairs = hs[5];
+    ctx.stats.total_cells = hs[6];
+    ctx.stats.key_bits = static_cast<int32_t>(hc[SS_BITS]) - box_sub_bits(D);
+    ctx.stats.occupied_cells = hc[SS_M];
+    ctx.stats.neighbor_rows = hc[SS_RANGES];
+    ctx.stats.boxes = hc[SS_NBOX];
+    ctx.stats.box_union = 1;
+    return true;
+}
+
 // A batch of jobs: rows [offsets[j], offsets[j+1]) of `pts` form job j with
 // its own eps[j] and min_pts[j].  A plain call is a batch of one.
 template <typename T, int D>
 static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offsets,
                       int32_t njobs, const float* eps, const uint64_t* min_pts, int64_t* labels,
                       hipStream_t s) {
+    if constexpr (std::is_floating_point<T>::value) {
+        // the static pipeline, captured once per shape and replayed; it hands
+        // the run back when one of its flags says so
+        if (ctx.graph && ctx.border_rule == HPD_BORDER_MIN_CLUSTER && ctx.main_use == &ctx.main &&
+            !ctx.timing &&
+            run_static<T, D>(ctx, static_cast<int>(sizeof(T)), pts, n, offsets, njobs, eps, min_pts,
+                             labels, s))
+            return;
+    }
     PhaseTimer tm(ctx, s);
     tm.mark(M_START);
     Arena& A = *ctx.main_use;
@@ -1025,6 +1445,7 @@ int hpd_context_destroy(hpd_context* ctx) {
         {
             hpd::DeviceGuard g(ctx->impl.device);
             if (ctx->impl.stream) (void)hipStreamSynchronize(ctx->impl.stream);
+            if (ctx->impl.graph_entry.exec) (void)hipGraphExecDestroy(ctx->impl.graph_entry.exec);
             for (auto e : ctx->impl.events) (void)hipEventDestroy(e);
             if (ctx->impl.host_scratch) (void)hipHostFree(ctx->impl.host_scratch);
             if (ctx->impl.stream) (void)hipStreamDestroy(ctx->impl.stream);
@@ -1046,6 +1467,30 @@ int hpd_context_set_timing(hpd_context* ctx, int enabled) {
     return guarded([&] {
         if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
         ctx->impl.timing = enabled != 0;
+    });
+}
+
+int hpd_context_set_graph(hpd_context* ctx, int enabled) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        Context& c = ctx->impl;
+        c.graph = enabled != 0;
+        if (!c.graph && c.graph_entry.exec) {
+            hpd::DeviceGuard g(c.device);
+            HPD_HIP(hipStreamSynchronize(c.stream));
+            (void)hipGraphExecDestroy(c.graph_entry.exec);
+            c.graph_entry = Context::GraphEntry{};
+        }
+    });
+}
+
+int hpd_context_get_graph_stats(const hpd_context* ctx, uint64_t* launches, uint64_t* captures,
+                                uint64_t* fallbacks) {
+    return guarded([&] {
+        if (!ctx || !launches || !captures || !fallbacks) throw Error(HPD_EINVAL, "NULL argument");
+        *launches = ctx->impl.graph_launches;
+        *captures = ctx->impl.graph_captures;
+        *fallbacks = ctx->impl.graph_fallbacks;
     });
 }
 

@@ -64,12 +64,10 @@ struct Chunk {
 
 __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int32_t nitems,
                                             const int32_t* __restrict__ ustart,
-                                            const uint32_t* __restrict__ roff, Chunk& c,
-                                            const unsigned int* nitems_dev = nullptr) {
+                                            const uint32_t* __restrict__ roff, Chunk& c) {
     const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                 static_cast<int32_t>(threadIdx.x >> 6));
     if (wid >= nitems) return false;
-    if (nitems_dev && wid >= uniform(static_cast<int32_t>(*nitems_dev))) return false;
     const int2 it = items[wid];
     c.u = uniform(it.x);
     c.first = uniform(it.y);
@@ -77,6 +75,34 @@ __device__ __forceinline__ bool fetch_chunk(const int2* __restrict__ items, int3
     c.r0 = uniform(static_cast<int32_t>(roff[c.u]));
     c.r1 = uniform(static_cast<int32_t>(roff[c.u + 1]));
     return true;
+}
+
+// Chunk of work item wid (no bounds check).
+__device__ __forceinline__ void chunk_at(const int2* __restrict__ items, int32_t wid,
+                                         const int32_t* __restrict__ ustart,
+                                         const uint32_t* __restrict__ roff, Chunk& c) {
+    const int2 it = items[wid];
+    c.u = uniform(it.x);
+    c.first = uniform(it.y);
+    c.end = uniform(ustart[c.u + 1]);
+    c.r0 = uniform(static_cast<int32_t>(roff[c.u]));
+    c.r1 = uniform(static_cast<int32_t>(roff[c.u + 1]));
+}
+
+// Number of work items: host-known, or written by an earlier kernel.
+__device__ __forceinline__ int32_t item_count(int32_t host, const unsigned int* dev) {
+    return dev ? uniform(static_cast<int32_t>(*dev)) : host;
+}
+
+// The work item of the calling wave: one wave per item where the grid covers
+// the host-known count or its bound (waves past a device-written count
+// exit); the waves of a smaller grid walk items wid, wid + wave_stride(), ...
+__device__ __forceinline__ int32_t first_wave() {
+    return uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                   static_cast<int32_t>(threadIdx.x >> 6));
+}
+__device__ __forceinline__ int32_t wave_stride() {
+    return static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
 }
 
 // The boxes of chunk c's neighbour rows (for_each_row_box).
@@ -152,11 +178,11 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
 // few ulps of d^2, so gmin2 > EPS2 (1 + 1e-5) (+ 1e-30) rules every member
 // out and gmax2 <= EPS2 (1 - 1e-5) rules every member in.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> a,
-                                                         uint8_t* __restrict__ core,
-                                                         unsigned long long* __restrict__ ncore) {
+__device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t* __restrict__ core,
+                                                unsigned long long* __restrict__ ncore,
+                                                int32_t wid) {
     Chunk c;
-    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c, a.nditems_dev)) return;
+    chunk_at(a.ditems, wid, a.ustart, a.roff, c);
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const uint32_t min_pts = jp.min_pts;
@@ -249,6 +275,27 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
     }
 }
 
+// One wave per work item.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> a,
+                                                         uint8_t* __restrict__ core,
+                                                         unsigned long long* __restrict__ ncore) {
+    const int32_t wid = first_wave();
+    if (wid < item_count(a.nditems, a.nditems_dev)) core_boxes_item<T, D>(a, core, ncore, wid);
+}
+
+// The waves walk the items: for a bound on a device-written count too large
+// for one wave per item.  The walk keeps the arguments live across items,
+// more than the 64 VGPRs of 8 waves per SIMD hold without spilling, so it
+// runs at 4.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads, 4) void k_core_boxes_walk(
+    const DbscanArgs<T> a, uint8_t* __restrict__ core, unsigned long long* __restrict__ ncore) {
+    const int32_t nwork = item_count(a.nditems, a.nditems_dev);
+    for (int32_t wid = first_wave(); wid < nwork; wid += wave_stride())
+        core_boxes_item<T, D>(a, core, ncore, wid);
+}
+
 // Thread-per-row core counts for the rows of sparse cells (a.sitems): the
 // same box-by-box count as k_core_boxes, with every thread walking its own
 // row's neighbourhood.
@@ -258,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_core_sparse(const DbscanArgs<T> a,
                                                           unsigned long long* __restrict__ ncore) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     bool is_core = false;
-    if (i < a.nsitems) {
+    if (i < item_count(a.nsitems, a.nsitems_dev)) {
         const int2 it = a.sitems[i];
         const int32_t p = it.x, u = it.y;
         const int32_t b = a.box_of_slot[p];
@@ -591,16 +638,13 @@ __global__ __launch_bounds__(kThreads) void k_sum_stripes(const unsigned long lo
 }
 
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T> a,
-                                                           const uint8_t* __restrict__ core,
-                                                           const int32_t* __restrict__ bcluster,
-                                                           const uint32_t* __restrict__ order,
-                                                           int64_t* __restrict__ labels,
-                                                           unsigned long long* __restrict__ stats,
-                                                           int2* __restrict__ amb_list,
-                                                           unsigned long long* __restrict__ noise) {
+__device__ __forceinline__ void border_boxes_item(
+    const DbscanArgs<T>& a, const uint8_t* __restrict__ core, const int32_t* __restrict__ bcluster,
+    const uint32_t* __restrict__ order, int64_t* __restrict__ labels,
+    unsigned long long* __restrict__ stats, int2* __restrict__ amb_list,
+    unsigned long long* __restrict__ noise, int32_t wid) {
     Chunk c;
-    if (!fetch_chunk(a.ditems, a.nditems, a.ustart, a.roff, c)) return;
+    chunk_at(a.ditems, wid, a.ustart, a.roff, c);
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const double gap2_max = jp.gap2_max;
@@ -655,6 +699,29 @@ __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(const DbscanArgs<T
     }
 }
 
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads, 8) void k_border_boxes(
+    const DbscanArgs<T> a, const uint8_t* __restrict__ core, const int32_t* __restrict__ bcluster,
+    const uint32_t* __restrict__ order, int64_t* __restrict__ labels,
+    unsigned long long* __restrict__ stats, int2* __restrict__ amb_list,
+    unsigned long long* __restrict__ noise) {
+    const int32_t wid = first_wave();
+    if (wid < item_count(a.nditems, a.nditems_dev))
+        border_boxes_item<T, D>(a, core, bcluster, order, labels, stats, amb_list, noise, wid);
+}
+
+// k_border_boxes with the waves walking the items (as k_core_boxes_walk).
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads, 4) void k_border_boxes_walk(
+    const DbscanArgs<T> a, const uint8_t* __restrict__ core, const int32_t* __restrict__ bcluster,
+    const uint32_t* __restrict__ order, int64_t* __restrict__ labels,
+    unsigned long long* __restrict__ stats, int2* __restrict__ amb_list,
+    unsigned long long* __restrict__ noise) {
+    const int32_t nwork = item_count(a.nditems, a.nditems_dev);
+    for (int32_t wid = first_wave(); wid < nwork; wid += wave_stride())
+        border_boxes_item<T, D>(a, core, bcluster, order, labels, stats, amb_list, noise, wid);
+}
+
 // Thread-per-row border labels for the non-core rows of sparse cells
 // (a.sitems); same box-by-box search as k_border_boxes.
 template <typename T, int D>
@@ -668,7 +735,7 @@ __global__ __launch_bounds__(kThreads) void k_border_sparse(const DbscanArgs<T> 
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     bool active = false, amb = false;
     int32_t best = INT_MAX, p = 0, u = 0;
-    if (s < a.nsitems) {
+    if (s < item_count(a.nsitems, a.nsitems_dev)) {
         const int2 it = a.sitems[s];
         p = it.x;
         u = it.y;
@@ -907,6 +974,11 @@ __global__ __launch_bounds__(kThreads) void k_fill2(int32_t* __restrict__ a, int
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
+// Up to this many work items get one wave each; a larger bound on a
+// device-written count launches this many waves of a walking kernel.
+constexpr int64_t kMaxItemWaves = int64_t{1} << 24;
+
+
 }  // namespace
 
 // *out += number of nonzero flags among f[0, n)
@@ -936,6 +1008,7 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
             if (!a.sparse_core) {  // the chunks of every cell that can hold cores
                 ac.ditems = a.citems ? a.citems : a.items;
                 ac.nditems = a.citems ? a.ncitems : a.nitems;
+                ac.nditems_dev = a.citems ? a.ncitems_dev : nullptr;
                 ac.nsitems = 0;
             }
             if (a.citems && !a.sparse_core && a.citems_need && a.bmincore) {
@@ -944,14 +1017,21 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
                 // core); only the chunks holding another row are counted
                 launch_core_full(a.n, a.box_of_slot, a.btight, a.bstart, a.inrow, core, a.bmincore,
                                  s);
-                launch_select_core_chunks(a.ncitems, a.citems, a.ustart, a.box_of_slot, a.btight,
-                                          a.citems_need, a.ncitems_need_dev, s);
+                launch_select_core_chunks(Count{a.ncitems, a.ncitems_dev}, a.citems, a.ustart,
+                                          a.box_of_slot, a.btight, a.citems_need,
+                                          a.ncitems_need_dev, s);
                 ac.ditems = a.citems_need;
                 ac.nditems_dev = a.ncitems_need_dev;
             } else if (a.citems && !a.sparse_core) {  // rows of the other cells are not core
                 HPD_HIP(hipMemsetAsync(core, 0, static_cast<size_t>(a.n), s));
             }
-            if (ac.nditems > 0)
+            // one wave per chunk of the bound (waves past the device-written
+            // count of needing chunks exit at once), or a walk
+            if (ac.nditems > kMaxItemWaves)
+                hipLaunchKernelGGL((k_core_boxes_walk<T, D>),
+                                   dim3(blocks_for(kMaxItemWaves, kWavesPerBlock)), dim3(kThreads),
+                                   0, s, ac, core, ncore);
+            else if (ac.nditems > 0)
                 hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(ac.nditems, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, ac, core, ncore);
             if (ac.nsitems > 0)
@@ -1019,11 +1099,15 @@ void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     if constexpr (std::is_floating_point<T>::value) {
         if (bcluster && a.bbox) {
-            if (a.nditems > 0)
-                hipLaunchKernelGGL((k_border_boxes<T, D>),
-                                   dim3(blocks_for(a.nditems, kWavesPerBlock)), dim3(kThreads), 0,
-                                   s, a, core, bcluster, a.inrow, labels, stats, amb_list,
+            if (a.nditems > kMaxItemWaves)
+                hipLaunchKernelGGL((k_border_boxes_walk<T, D>),
+                                   dim3(blocks_for(kMaxItemWaves, kWavesPerBlock)), dim3(kThreads),
+                                   0, s, a, core, bcluster, a.inrow, labels, stats, amb_list,
                                    noise_stripes);
+            else if (a.nditems > 0)
+                hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(blocks_for(a.nditems, kWavesPerBlock)),
+                                   dim3(kThreads), 0, s, a, core, bcluster, a.inrow, labels, stats,
+                                   amb_list, noise_stripes);
             if (a.nsitems > 0)
                 hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, a, core, bcluster, labels, stats,

@@ -44,6 +44,8 @@ struct DbscanArgs {
     int32_t nditems = 0;
     const int2* sitems = nullptr;
     int32_t nsitems = 0;
+    const uint32_t* nsitems_dev = nullptr;  // static pipeline: nsitems in device memory
+    const uint32_t* ncitems_dev = nullptr;  // static pipeline: ncitems in device memory
     bool sparse_core = false;  // core count split too (else one wave per chunk of every cell)
     // chunks of the cells with >= minPts candidates: the only cells that can
     // hold cores (core flags of the others stay 0; null: every chunk)

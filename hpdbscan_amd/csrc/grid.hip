@@ -8,6 +8,7 @@
 //                                      array per dimension
 #include <cfloat>
 #include <limits>
+#include <type_traits>
 
 #include "common.hpp"
 #include "grid.hpp"
@@ -277,9 +278,140 @@ __global__ __launch_bounds__(256) void k_gather_soa(const T* __restrict__ pts, i
     for (int j = 0; j < D; ++j) soa[static_cast<int64_t>(j) * ld + i] = pts[o * D + j];
 }
 
+// float / double -> integer as x86-64 cvttss2si / cvttsd2si produce it
+// (out of range or NaN: the "integer indefinite" value)
+__device__ __forceinline__ int64_t d2i64_x86(double v) {
+    if (!(v >= -9223372036854775808.0 && v < 9223372036854775808.0)) return INT64_MIN;
+    return static_cast<int64_t>(v);
+}
+__device__ __forceinline__ uint64_t d2u64_x86(double v) {
+    if (v < 9223372036854775808.0) return static_cast<uint64_t>(d2i64_x86(v));
+    return static_cast<uint64_t>(d2i64_x86(v - 9223372036854775808.0)) ^ 0x8000000000000000ull;
+}
+
+// cells of one dimension, ceil((max - min) / eps) + 1 (spatial_index.h:124-131)
+// with the reciprocal form and the flushing of the reference's build (device
+// code flushes denormal inputs and results itself)
+template <typename T>
+__device__ __forceinline__ uint64_t cells_of_dev(T mx, T mn, float rcp_f, double rcp_d) {
+    if constexpr (std::is_same<T, float>::value) {
+        const float v = (mx - mn) * rcp_f;
+        return d2u64_x86(static_cast<double>(ceilf(v))) + 1;
+    } else if constexpr (std::is_same<T, double>::value) {
+        const double v = (mx - mn) * rcp_d;
+        return d2u64_x86(ceil(v)) + 1;
+    } else {
+        const float v = int_diff_f32<T>(mx, mn) * rcp_f;
+        return d2u64_x86(static_cast<double>(ceilf(v))) + 1;
+    }
+}
+
+// The grid of every job from its min / max (mm: 2D values per job), as the
+// host computes it after reading them back: cells per dimension, dimension
+// order (an insertion sort ascending by cells, which is what std::sort does
+// for so few keys: ties keep their order), strides; then key bases in job
+// order.  flags[0] |= 1: a grid of more than 2^64 cells; *sort_bits = bits
+// of the largest key + sub_bits; *total = all jobs' cells.  One block.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_geometry(JobParams<T>* __restrict__ jobs, int32_t njobs,
+                                                  const T* __restrict__ mm, int sub_bits,
+                                                  uint64_t* __restrict__ cells,
+                                                  int* __restrict__ flags,
+                                                  int* __restrict__ sort_bits,
+                                                  unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long part[256];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    // per job: geometry and number of cells (0 for a job without rows)
+    for (int32_t j = threadIdx.x; j < njobs; j += 256) {
+        JobParams<T>& jp = jobs[j];
+        uint64_t tot = 0;
+        if (jp.row_end > jp.row_begin) {
+            const T* mn = &mm[static_cast<int64_t>(j) * 2 * D];
+            uint64_t c[D];
+            tot = 1;
+            bool of = false;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                c[k] = cells_of_dev<T>(mn[D + k], mn[k], jp.rcp_f, jp.rcp_d);
+                uint64_t t;
+                if (__builtin_mul_overflow(tot, c[k], &t)) of = true;
+                tot = t;
+            }
+            if (of) atomicOr(&bad, 1);
+            int sw[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) sw[k] = k;
+            for (int k = 1; k < D; ++k) {
+                const int v = sw[k];
+                int i = k - 1;
+                while (i >= 0 && c[v] < c[sw[i]]) {
+                    sw[i + 1] = sw[i];
+                    --i;
+                }
+                sw[i + 1] = v;
+            }
+            uint64_t acc = 1;
+            jp.cp.d = D;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                jp.minv[k] = mn[k];
+                jp.cp.swapped[k] = sw[k];
+                jp.cp.stride[k] = acc;
+                jp.cp.ncell[k] = c[sw[k]];
+                acc *= c[sw[k]];
+            }
+        }
+        cells[j] = tot;
+    }
+    __syncthreads();
+    // key bases: each thread sums a contiguous run of jobs, the runs are
+    // scanned, then every thread assigns its run's bases
+    const int32_t per = (njobs + 255) / 256;
+    const int32_t j0 = static_cast<int32_t>(threadIdx.x) * per;
+    const int32_t j1 = min(njobs, j0 + per);
+    unsigned long long sum = 0;
+    bool of = false;
+    for (int32_t j = j0; j < j1; ++j) of |= __builtin_add_overflow(sum, cells[j], &sum);
+    part[threadIdx.x] = sum;
+    if (of) atomicOr(&bad, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long acc = 0;
+        bool o2 = false;
+        for (int t = 0; t < 256; ++t) {
+            const unsigned long long v = part[t];
+            part[t] = acc;
+            o2 |= __builtin_add_overflow(acc, v, &acc);
+        }
+        if (o2) bad = 1;
+        *total = acc;
+        const uint64_t maxkey = acc > 0 ? acc - 1 : 0;
+        const int kb = maxkey == 0 ? 0 : 64 - __clzll(static_cast<long long>(maxkey));
+        *sort_bits = kb + sub_bits;
+        if (kb + sub_bits > 64) bad = 1;
+    }
+    __syncthreads();
+    unsigned long long base = part[threadIdx.x];
+    for (int32_t j = j0; j < j1; ++j) {
+        jobs[j].key_base = base;
+        base += cells[j];
+    }
+    if (threadIdx.x == 0 && bad) flags[0] |= 1;
+}
+
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
 }  // namespace
+
+template <typename T, int D>
+void launch_geometry(JobParams<T>* jobs, int32_t njobs, const T* mm, int sub_bits, uint64_t* cells,
+                     int* flags, int* sort_bits, unsigned long long* total, hipStream_t s) {
+    hipLaunchKernelGGL((k_geometry<T, D>), dim3(1), dim3(256), 0, s, jobs, njobs, mm, sub_bits, cells,
+                       flags, sort_bits, total);
+    HPD_HIP(hipGetLastError());
+}
 
 int minmax_rows_per_block() { return kMMRows; }
 
@@ -312,6 +444,8 @@ void launch_gather_soa(const T* pts, int64_t n, const uint32_t* order, T* soa, i
 }
 
 #define HPD_INST_GRID(T, D)                                                                     \
+    template void launch_geometry<T, D>(JobParams<T>*, int32_t, const T*, int, uint64_t*, int*,  \
+                                        int*, unsigned long long*, hipStream_t);                  \
     template void launch_minmax<T, D>(const T*, const JobParams<T>*, int32_t, const int32_t*,    \
                                       int32_t, T*, T*, int*, hipStream_t);                       \
     template void launch_cell_ids<T, D, uint32_t>(const T*, int64_t, const JobParams<T>*,        \

@@ -26,6 +26,15 @@ namespace hpd {
 // rows reduced by one block of the min/max kernel
 int minmax_rows_per_block();
 
+// The static pipeline's grid of every job, on the device: from the jobs'
+// min / max (mm) it fills cp, minv and key_base of every job (host-written
+// row ranges, rcp_f, rcp_d), cells[j] = job j's cells, *sort_bits, *total;
+// flags[0] |= 1 when the grids cannot be keyed in 64 bits (the host path
+// then takes over).  Same arithmetic as the host's (make_geometry).
+template <typename T, int D>
+void launch_geometry(JobParams<T>* jobs, int32_t njobs, const T* mm, int sub_bits, uint64_t* cells,
+                     int* flags, int* sort_bits, unsigned long long* total, hipStream_t s);
+
 // Per job j: out[j*2D .. j*2D+D) = min, out[j*2D+D .. (j+1)*2D) = max over
 // the job's rows.  bfirst[j] = first partial block of job j (host-made,
 // ceil(rows / minmax_rows_per_block()) blocks per job, bfirst[njobs] =

@@ -145,7 +145,7 @@ __device__ __forceinline__ int32_t job_of_key(const JobParams<T>* __restrict__ j
 template <typename K, typename T>
 __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict__ ucell,
                                                              const int32_t* __restrict__ ustart,
-                                                             int32_t m,
+                                                             Count mc,
                                                              const JobParams<T>* __restrict__ jobs,
                                                              int32_t njobs,
                                                              int32_t* __restrict__ job_of_cell,
@@ -157,6 +157,7 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              int32_t stash_rows,
                                                              int32_t* __restrict__ center_local,
                                                              bool by_class) {
+    const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
     uint64_t count_u = 0;
@@ -209,13 +210,14 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
 template <typename K, typename T>
 __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict__ ucell,
                                                             const int32_t* __restrict__ ustart,
-                                                            int32_t m,
+                                                            Count mc,
                                                             const JobParams<T>* __restrict__ jobs,
                                                             const int32_t* __restrict__ job_of_cell,
                                                             const uint32_t* __restrict__ roff,
                                                             int2* __restrict__ ranges,
                                                             int2* __restrict__ rcells,
                                                             int32_t* __restrict__ rcenter) {
+    const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const int32_t j = job_of_cell[u];
@@ -321,8 +323,9 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_copy_class(const int2* __
     rcenter[u] = static_cast<int32_t>(w0);  // the own row is the only one of class 0
 }
 
-__global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, int32_t m,
+__global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ ustart, Count mc,
                                                       uint32_t* __restrict__ nchunks) {
+    const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const int32_t c = ustart[u + 1] - ustart[u];
@@ -333,9 +336,11 @@ __global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict_
 // chunk: the group is the last one whose chunk offset is <= w (binary search
 // over the exclusive scan off[0 .. ngroups]; empty groups are skipped)
 __global__ __launch_bounds__(256) void k_expand_chunks(const uint32_t* __restrict__ off,
-                                                       int32_t ngroups,
+                                                       Count ngroups_c,
                                                        const int32_t* __restrict__ start,
-                                                       int32_t total, int2* __restrict__ items) {
+                                                       Count total_c, int2* __restrict__ items) {
+    const int32_t ngroups = static_cast<int32_t>(ngroups_c.get());
+    const int32_t total = static_cast<int32_t>(total_c.get());
     const int32_t w = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (w >= total) return;
     int32_t lo = 0, hi = ngroups;  // off[lo] <= w < off[hi]
@@ -349,10 +354,11 @@ __global__ __launch_bounds__(256) void k_expand_chunks(const uint32_t* __restric
 
 // Per cell: 64-row chunks if the cell has more than `sparse_max` rows
 // (dense), else its rows one by one (sparse).
-__global__ __launch_bounds__(256) void k_split_counts(const int32_t* __restrict__ ustart, int32_t m,
+__global__ __launch_bounds__(256) void k_split_counts(const int32_t* __restrict__ ustart, Count mc,
                                                       int32_t sparse_max,
                                                       uint32_t* __restrict__ dcnt,
                                                       uint32_t* __restrict__ scnt) {
+    const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const int32_t c = ustart[u + 1] - ustart[u];
@@ -361,12 +367,13 @@ __global__ __launch_bounds__(256) void k_split_counts(const int32_t* __restrict_
     scnt[u] = sparse ? static_cast<uint32_t>(c) : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_split_items(const int32_t* __restrict__ ustart, int32_t m,
+__global__ __launch_bounds__(256) void k_split_items(const int32_t* __restrict__ ustart, Count mc,
                                                      int32_t sparse_max,
                                                      const uint32_t* __restrict__ doff,
                                                      const uint32_t* __restrict__ soff,
                                                      int2* __restrict__ ditems,
                                                      int2* __restrict__ sitems) {
+    const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
     const int32_t s = ustart[u], e = ustart[u + 1];
@@ -380,27 +387,29 @@ __global__ __launch_bounds__(256) void k_split_items(const int32_t* __restrict__
 
 }  // namespace
 
-void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, uint32_t* dcnt,
+void launch_split_counts(const int32_t* ustart, Count m, int32_t sparse_max, uint32_t* dcnt,
                          uint32_t* scnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_split_counts, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+    if (m.host <= 0) return;
+    hipLaunchKernelGGL(k_split_counts, dim3(blocks_for(m.host, 256)), dim3(256), 0, s, ustart, m,
                        sparse_max, dcnt, scnt);
     HPD_HIP(hipGetLastError());
 }
 
-void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
-                        const uint32_t* soff, int2* ditems, int32_t nditems, int2* sitems,
+void launch_split_items(const int32_t* ustart, Count m, int32_t sparse_max, const uint32_t* doff,
+                        const uint32_t* soff, int2* ditems, Count nditems, int2* sitems,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_split_items, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+    if (m.host <= 0) return;
+    hipLaunchKernelGGL(k_split_items, dim3(blocks_for(m.host, 256)), dim3(256), 0, s, ustart, m,
                        sparse_max, doff, soff, ditems, sitems);
     launch_expand_chunks(doff, m, ustart, nditems, ditems, s);
     HPD_HIP(hipGetLastError());
 }
 
-void launch_expand_chunks(const uint32_t* off, int32_t ngroups, const int32_t* start,
-                          int32_t total, int2* items, hipStream_t s) {
-    if (total <= 0) return;
-    hipLaunchKernelGGL(k_expand_chunks, dim3(blocks_for(total, 256)), dim3(256), 0, s, off, ngroups,
-                       start, total, items);
+void launch_expand_chunks(const uint32_t* off, Count ngroups, const int32_t* start, Count total,
+                          int2* items, hipStream_t s) {
+    if (total.host <= 0) return;
+    hipLaunchKernelGGL(k_expand_chunks, dim3(blocks_for(total.host, 256)), dim3(256), 0, s, off,
+                       ngroups, start, total, items);
     HPD_HIP(hipGetLastError());
 }
 
@@ -418,12 +427,13 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 }
 
 template <typename K, typename T>
-void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, Count m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
                            unsigned long long* rows_total, int2* stash, int32_t stash_rows,
                            int32_t* rcenter, bool by_class, hipStream_t s) {
-    hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
+    if (m.host <= 0) return;
+    hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m.host, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
                        rows_total, stash, stash_rows, rcenter, by_class);
     HPD_HIP(hipGetLastError());
@@ -446,22 +456,24 @@ void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* 
 }
 
 template <typename K, typename T>
-void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m,
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, Count m,
                           const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
                           hipStream_t s) {
-    hipLaunchKernelGGL((k_neighbor_rows_fill<K, T>), dim3(blocks_for(m, 256)), dim3(256), 0, s,
+    if (m.host <= 0) return;
+    hipLaunchKernelGGL((k_neighbor_rows_fill<K, T>), dim3(blocks_for(m.host, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, job_of_cell, roff, ranges, rcells, rcenter);
     HPD_HIP(hipGetLastError());
 }
 
-void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s) {
-    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(m, 256)), dim3(256), 0, s, ustart, m,
+void launch_chunk_counts(const int32_t* ustart, Count m, uint32_t* nchunks, hipStream_t s) {
+    if (m.host <= 0) return;
+    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(m.host, 256)), dim3(256), 0, s, ustart, m,
                        nchunks);
     HPD_HIP(hipGetLastError());
 }
 
-void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int32_t nitems,
+void launch_work_items(const int32_t* ustart, Count m, const uint32_t* coff, Count nitems,
                        int2* items, hipStream_t s) {
     launch_expand_chunks(coff, m, ustart, nitems, items, s);
 }
@@ -473,21 +485,21 @@ HPD_INST_INDEX(uint32_t)
 HPD_INST_INDEX(uint64_t)
 
 #define HPD_INST_ROWS(T)                                                                        \
-    template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,  \
+    template void launch_neighbor_count<uint32_t, T>(const uint32_t*, const int32_t*, Count,    \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
                                                      unsigned long long*, int2*, int32_t,       \
                                                      int32_t*, bool, hipStream_t);              \
-    template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,  \
+    template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, Count,    \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
                                                      unsigned long long*, int2*, int32_t,       \
                                                      int32_t*, bool, hipStream_t);              \
-    template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, int32_t,   \
+    template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, Count,     \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \
                                                     hipStream_t);                               \
-    template void launch_neighbor_fill<uint64_t, T>(const uint64_t*, const int32_t*, int32_t,   \
+    template void launch_neighbor_fill<uint64_t, T>(const uint64_t*, const int32_t*, Count,     \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \
                                                     hipStream_t);

@@ -27,7 +27,7 @@ void launch_cells(const K* keys, int64_t n, int shift, uint32_t* flags, uint32_t
 // each row's offset class (the number of dimensions it is offset in) rides
 // in the top bits of its two cell indices.
 template <typename K, typename T>
-void launch_neighbor_count(const K* ucell, const int32_t* ustart, int32_t m,
+void launch_neighbor_count(const K* ucell, const int32_t* ustart, Count m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
                            unsigned long long* rows_total, int2* stash, int32_t stash_rows,
@@ -44,28 +44,28 @@ void launch_neighbor_copy(const int2* stash, int32_t stash_rows, const int32_t* 
 // ascending; rcells[..] = [first, end) occupied-cell index of the same row;
 // rcenter[u] = index (into ranges) of the row holding cell u itself
 template <typename K, typename T>
-void launch_neighbor_fill(const K* ucell, const int32_t* ustart, int32_t m,
+void launch_neighbor_fill(const K* ucell, const int32_t* ustart, Count m,
                           const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
                           hipStream_t s);
 
-void launch_chunk_counts(const int32_t* ustart, int32_t m, uint32_t* nchunks, hipStream_t s);
+void launch_chunk_counts(const int32_t* ustart, Count m, uint32_t* nchunks, hipStream_t s);
 
 // Work split of the neighbour kernels: cells with more than sparse_max rows
 // give one item (cell, first row) per 64-row chunk (dcnt: chunks per cell),
 // the others one item (row, cell) per row (scnt: rows per cell).  doff /
 // soff are the exclusive scans of dcnt / scnt.
-void launch_split_counts(const int32_t* ustart, int32_t m, int32_t sparse_max, uint32_t* dcnt,
+void launch_split_counts(const int32_t* ustart, Count m, int32_t sparse_max, uint32_t* dcnt,
                          uint32_t* scnt, hipStream_t s);
-void launch_split_items(const int32_t* ustart, int32_t m, int32_t sparse_max, const uint32_t* doff,
-                        const uint32_t* soff, int2* ditems, int32_t nditems, int2* sitems,
+void launch_split_items(const int32_t* ustart, Count m, int32_t sparse_max, const uint32_t* doff,
+                        const uint32_t* soff, int2* ditems, Count nditems, int2* sitems,
                         hipStream_t s);
 
 // items[w] = (g, start[g] + 64 * (w - off[g])) for the `total` 64-row chunks
 // of ngroups groups (off: exclusive scan of chunk counts, ngroups + 1 entries)
-void launch_expand_chunks(const uint32_t* off, int32_t ngroups, const int32_t* start,
-                          int32_t total, int2* items, hipStream_t s);
-void launch_work_items(const int32_t* ustart, int32_t m, const uint32_t* coff, int32_t nitems,
+void launch_expand_chunks(const uint32_t* off, Count ngroups, const int32_t* start, Count total,
+                          int2* items, hipStream_t s);
+void launch_work_items(const int32_t* ustart, Count m, const uint32_t* coff, Count nitems,
                        int2* items, hipStream_t s);
 
 }  // namespace hpd

@@ -50,6 +50,7 @@ public:
         return static_cast<T*>(take(n * sizeof(T)));
     }
     size_t capacity() const { return cap_; }
+    const void* base() const { return base_; }
     size_t used() const { return used_; }
 
 private:
@@ -61,12 +62,31 @@ private:
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// A number of items known on the host, or (dev not null) written into device
+// memory by an earlier kernel, `host` being then an upper bound that sizes
+// the launch.  The static pipeline (no host read-backs) passes the latter.
+struct Count {
+    int64_t host = 0;
+    const uint32_t* dev = nullptr;
+    __device__ __forceinline__ int64_t get() const {
+        return dev ? static_cast<int64_t>(*dev) : host;
+    }
+};
+inline Count host_count(int64_t n) { return Count{n, nullptr}; }
+
 // Exclusive prefix sum of n uint32 values (in may equal out).  `tmp` must
 // hold scan_tmp_words(n) uint32.  The total is written to *total_dev when
 // total_dev is not null (device memory).
 size_t scan_tmp_words(int64_t n);
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                         uint32_t* total_dev, hipStream_t s);
+// The same over a length that an earlier kernel wrote (n.dev, at most
+// n.host, which sizes the launches); the total goes to out[length].
+void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, Count n, uint32_t* tmp,
+                            hipStream_t s);
+// *dst = a[*idx_dev] (a scan's total at a device-written length, moved to a
+// fixed place)
+void copy_at_count(const uint32_t* a, const uint32_t* idx_dev, uint32_t* dst, hipStream_t s);
 
 // In-place exclusive scans of k <= kMaxScanSet arrays of one length n, each
 // array's total to its entry n (every array holds n + 1 entries), in two
@@ -77,13 +97,20 @@ struct ScanSet {
     uint32_t* a[kMaxScanSet];
 };
 void exclusive_scan_set_u32(const ScanSet& set, int k, int64_t n, uint32_t* tmp, size_t tmp_words,
-                            hipStream_t s);
+                            hipStream_t s, const uint32_t* n_dev = nullptr);
 
 // Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the
 // key.  Returns true when the result is in (keys_b, vals_b), false when it is
 // in (keys_a, vals_a).  `hist` must hold radix_hist_words(n) uint32 and `tmp`
 // scan_tmp_words(radix_hist_words(n)).
 size_t radix_hist_words(int64_t n);
+// The same with the key's significant bits in device memory (*bits_dev <=
+// max_bits, 8-bit digits; passes beyond the bits do nothing): the result is
+// always in (keys_a, vals_a).
+template <typename K>
+void radix_sort_pairs_dev(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                          int max_bits, const int* bits_dev, uint32_t* hist, uint32_t* tmp,
+                          hipStream_t s);
 template <typename K>
 bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
                       int bits, uint32_t* hist, uint32_t* tmp, hipStream_t s);

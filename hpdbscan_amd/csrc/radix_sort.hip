@@ -29,7 +29,9 @@ __host__ __device__ inline int64_t tiles_of(int64_t n) { return (n + kTile - 1) 
 template <typename K, int kBits>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ keys, int64_t n,
                                                          int shift, uint32_t* __restrict__ hist,
-                                                         int64_t ntiles) {
+                                                         int64_t ntiles,
+                                                         const int* __restrict__ bits_dev) {
+    if (bits_dev && shift >= *bits_dev) return;  // a pass beyond the key's bits
     constexpr int B = 1 << kBits;
     constexpr uint32_t kMask = B - 1;
     __shared__ uint32_t cnt[B];
@@ -58,7 +60,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
                                                             uint32_t* __restrict__ vout, int64_t n,
                                                             int shift,
                                                             const uint32_t* __restrict__ offs,
-                                                            int64_t ntiles) {
+                                                            int64_t ntiles,
+                                                            const int* __restrict__ bits_dev) {
+    if (bits_dev && shift >= *bits_dev) return;  // a pass beyond the key's bits
     constexpr int B = 1 << kBits;
     constexpr uint32_t kMask = B - 1;
     constexpr int kOwn = B / kThreads;  // consecutive digits per thread
@@ -169,9 +173,28 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     }
 }
 
+// After passes beyond a device-written bit count did nothing, the result of
+// the `launched` passes lies in the b buffers when an odd number of them ran:
+// copy it back to the a buffers.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_radix_settle(K* __restrict__ keys_a,
+                                                           uint32_t* __restrict__ vals_a,
+                                                           const K* __restrict__ keys_b,
+                                                           const uint32_t* __restrict__ vals_b,
+                                                           int64_t n, int digit_bits,
+                                                           const int* __restrict__ bits_dev) {
+    const int ran = (*bits_dev + digit_bits - 1) / digit_bits;
+    if ((ran & 1) == 0) return;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kThreads) {
+        keys_a[i] = keys_b[i];
+        vals_a[i] = vals_b[i];
+    }
+}
+
 template <typename K, int kBits>
 bool sort_passes(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n, int bits,
-                 uint32_t* hist, uint32_t* tmp, hipStream_t s) {
+                 uint32_t* hist, uint32_t* tmp, hipStream_t s, const int* bits_dev = nullptr) {
     const int64_t nt = tiles_of(n);
     const int64_t nh = static_cast<int64_t>(1 << kBits) * nt;
     K* kin = keys_a;
@@ -181,10 +204,11 @@ bool sort_passes(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64
     bool in_b = false;
     for (int shift = 0; shift < bits; shift += kBits) {
         hipLaunchKernelGGL((k_radix_hist<K, kBits>), dim3(static_cast<unsigned>(nt)),
-                           dim3(kThreads), 0, s, kin, n, shift, hist, nt);
+                           dim3(kThreads), 0, s, kin, n, shift, hist, nt, bits_dev);
         exclusive_scan_u32(hist, hist, nh, tmp, nullptr, s);
         hipLaunchKernelGGL((k_radix_scatter<K, kBits>), dim3(static_cast<unsigned>(nt)),
-                           dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift, hist, nt);
+                           dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift, hist, nt,
+                           bits_dev);
         HPD_HIP(hipGetLastError());
         K* tk = kin;
         kin = kout;
@@ -212,6 +236,22 @@ bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
         return sort_passes<K, 9>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
     return sort_passes<K, 8>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
 }
+
+template <typename K>
+void radix_sort_pairs_dev(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                          int max_bits, const int* bits_dev, uint32_t* hist, uint32_t* tmp,
+                          hipStream_t s) {
+    if (n <= 1 || max_bits <= 0) return;
+    sort_passes<K, 8>(keys_a, vals_a, keys_b, vals_b, n, max_bits, hist, tmp, s, bits_dev);
+    const int64_t want = (n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_radix_settle<K>, dim3(static_cast<unsigned>(want < 4096 ? want : 4096)),
+                       dim3(kThreads), 0, s, keys_a, vals_a, static_cast<const K*>(keys_b),
+                       static_cast<const uint32_t*>(vals_b), n, 8, bits_dev);
+    HPD_HIP(hipGetLastError());
+}
+
+template void radix_sort_pairs_dev<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t,
+                                             int, const int*, uint32_t*, uint32_t*, hipStream_t);
 
 template bool radix_sort_pairs<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
                                          uint32_t*, uint32_t*, hipStream_t);

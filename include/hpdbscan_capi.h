@@ -116,6 +116,24 @@ int hpd_context_set_border_rule(hpd_context* ctx, int rule);
 int hpd_context_set_timing(hpd_context* ctx, int enabled);
 int hpd_context_get_stats(const hpd_context* ctx, hpd_run_stats* out);
 
+/* Graph mode (no counterpart in the reference): with enabled != 0,
+ * floating-point runs of this context (plain and batched calls, smallest-
+ * cluster border rule) take the static pipeline: every table is sized by its
+ * worst case from the number of rows, jobs and dimensions and every count
+ * stays in device memory, so no host read-back happens between the first
+ * kernel and the last.  The sequence is captured once in a hipGraph and
+ * replayed by later calls with the same shape (dtype, d, rows, job offsets,
+ * data and label buffers, options); eps, minPts and the data itself may
+ * change.  The labels are identical to the plain pipeline's.  It uses more
+ * device memory (about 3^(d-1) * 16 + 260 bytes per row) and falls back to
+ * the plain pipeline, transparently, for loose sub-boxes, grids beyond 64
+ * key bits or too many open box pairs; non-finite input then fails as
+ * usual.  Disabling it frees the graph. */
+int hpd_context_set_graph(hpd_context* ctx, int enabled);
+/* Graph launches, captures and fallbacks to the plain pipeline so far. */
+int hpd_context_get_graph_stats(const hpd_context* ctx, uint64_t* launches, uint64_t* captures,
+                                uint64_t* fallbacks);
+
 /* Work split of the neighbour kernels on sub-boxes (floating-point data):
  * cells with at most sparse_max rows (0..64) are processed one GPU thread
  * per row in the border pass, and in the core count too when sparse_core is
