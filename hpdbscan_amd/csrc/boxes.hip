@@ -25,6 +25,7 @@
 // eps * (1 + 1e-5) cannot hold such a pair (the float result of the
 // expression is within a few ulps of the exact squared distance) and are
 // skipped.
+#include <algorithm>
 #include <climits>
 
 #include "boxes.hpp"
@@ -1004,12 +1005,16 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
                        a.bcell, a.job_of_cell, a.jobs, btight, nloose);
 }
 
-// a grid of one wave per work item of the count or its bound (the kernels'
-// wave loops then run once; waves past a device-written count exit), at
-// most kMaxItemWaves waves, which walk a larger count
+// A grid of waves for a count of work items: one wave per item of a
+// host-known count (the kernels' wave loops then run once), at most
+// kMaxItemWaves, which walk a larger count.  For a count an earlier kernel
+// wrote (the host holds a bound, n or n + n / 64) one wave per item up to an
+// eighth of the bound, which walk the items beyond it: every wave past the
+// count costs a dispatch.
 constexpr int64_t kMaxItemWaves = int64_t{1} << 24;
 inline unsigned wave_blocks(Count items) {
-    const int64_t w = items.host < kMaxItemWaves ? items.host : kMaxItemWaves;
+    int64_t w = items.dev ? std::min(items.host, items.host / 8 + 64) : items.host;
+    w = std::min(w, kMaxItemWaves);
     return blocks_for(w > 0 ? w : 1, kWavesPerBlock);
 }
 

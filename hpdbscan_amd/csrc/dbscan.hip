@@ -284,15 +284,16 @@ __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> 
     if (wid < item_count(a.nditems, a.nditems_dev)) core_boxes_item<T, D>(a, core, ncore, wid);
 }
 
-// The waves walk the items: for a bound on a device-written count too large
-// for one wave per item.  The walk keeps the arguments live across items,
-// more than the 64 VGPRs of 8 waves per SIMD hold without spilling, so it
-// runs at 4.
+// The waves walk items wid0, wid0 + 1, ...: the items of a device-written
+// count beyond what a one-wave-per-item grid covered.  The walk keeps the
+// arguments live across items, more than the 64 VGPRs of 8 waves per SIMD
+// hold without spilling, so it runs at 4.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads, 4) void k_core_boxes_walk(
-    const DbscanArgs<T> a, uint8_t* __restrict__ core, unsigned long long* __restrict__ ncore) {
+    const DbscanArgs<T> a, uint8_t* __restrict__ core, unsigned long long* __restrict__ ncore,
+    int32_t wid0) {
     const int32_t nwork = item_count(a.nditems, a.nditems_dev);
-    for (int32_t wid = first_wave(); wid < nwork; wid += wave_stride())
+    for (int32_t wid = wid0 + first_wave(); wid < nwork; wid += wave_stride())
         core_boxes_item<T, D>(a, core, ncore, wid);
 }
 
@@ -710,15 +711,15 @@ __global__ __launch_bounds__(kThreads, 8) void k_border_boxes(
         border_boxes_item<T, D>(a, core, bcluster, order, labels, stats, amb_list, noise, wid);
 }
 
-// k_border_boxes with the waves walking the items (as k_core_boxes_walk).
+// k_border_boxes with the waves walking items wid0, ... (as k_core_boxes_walk).
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads, 4) void k_border_boxes_walk(
     const DbscanArgs<T> a, const uint8_t* __restrict__ core, const int32_t* __restrict__ bcluster,
     const uint32_t* __restrict__ order, int64_t* __restrict__ labels,
     unsigned long long* __restrict__ stats, int2* __restrict__ amb_list,
-    unsigned long long* __restrict__ noise) {
+    unsigned long long* __restrict__ noise, int32_t wid0) {
     const int32_t nwork = item_count(a.nditems, a.nditems_dev);
-    for (int32_t wid = first_wave(); wid < nwork; wid += wave_stride())
+    for (int32_t wid = wid0 + first_wave(); wid < nwork; wid += wave_stride())
         border_boxes_item<T, D>(a, core, bcluster, order, labels, stats, amb_list, noise, wid);
 }
 
@@ -974,9 +975,25 @@ __global__ __launch_bounds__(kThreads) void k_fill2(int32_t* __restrict__ a, int
 
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
-// Up to this many work items get one wave each; a larger bound on a
-// device-written count launches this many waves of a walking kernel.
+// Waves for the chunk items of the core and border kernels.  A host-known
+// count gets one wave per item.  A count that an earlier kernel wrote (the
+// host holds a bound, n + n / 64 for chunks) gets one wave per item up to a
+// likely count, n / 8 + n / 64 (cells of 8 rows on average; every wave past
+// the count exits at once, and each costs a dispatch), and a small walking
+// grid for the items beyond it, which exits at once when there are none.
 constexpr int64_t kMaxItemWaves = int64_t{1} << 24;
+constexpr int64_t kTailWaves = 16384;
+struct ItemGrid {
+    int64_t waves = 0;  // one-wave-per-item grid: items [0, waves)
+    bool tail = false;  // a walking grid for items [waves, count)
+};
+inline ItemGrid item_grid(int64_t bound, bool device_count, int64_t n) {
+    ItemGrid g;
+    const int64_t likely = device_count ? n / 8 + n / 64 + 64 : bound;
+    g.waves = std::min(bound, std::min(likely, kMaxItemWaves));
+    g.tail = bound > g.waves;
+    return g;
+}
 
 
 }  // namespace
@@ -1025,15 +1042,14 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
             } else if (a.citems && !a.sparse_core) {  // rows of the other cells are not core
                 HPD_HIP(hipMemsetAsync(core, 0, static_cast<size_t>(a.n), s));
             }
-            // one wave per chunk of the bound (waves past the device-written
-            // count of needing chunks exit at once), or a walk
-            if (ac.nditems > kMaxItemWaves)
-                hipLaunchKernelGGL((k_core_boxes_walk<T, D>),
-                                   dim3(blocks_for(kMaxItemWaves, kWavesPerBlock)), dim3(kThreads),
-                                   0, s, ac, core, ncore);
-            else if (ac.nditems > 0)
-                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(ac.nditems, kWavesPerBlock)),
+            const ItemGrid g = item_grid(ac.nditems, ac.nditems_dev != nullptr, a.n);
+            if (g.waves > 0)
+                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(g.waves, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, ac, core, ncore);
+            if (g.tail)
+                hipLaunchKernelGGL((k_core_boxes_walk<T, D>),
+                                   dim3(blocks_for(kTailWaves, kWavesPerBlock)), dim3(kThreads), 0,
+                                   s, ac, core, ncore, static_cast<int32_t>(g.waves));
             if (ac.nsitems > 0)
                 hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(ac.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, ac, core, ncore);
@@ -1099,15 +1115,16 @@ void launch_border(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* b
     const unsigned gw = blocks_for(a.nitems, kWavesPerBlock);
     if constexpr (std::is_floating_point<T>::value) {
         if (bcluster && a.bbox) {
-            if (a.nditems > kMaxItemWaves)
-                hipLaunchKernelGGL((k_border_boxes_walk<T, D>),
-                                   dim3(blocks_for(kMaxItemWaves, kWavesPerBlock)), dim3(kThreads),
-                                   0, s, a, core, bcluster, a.inrow, labels, stats, amb_list,
-                                   noise_stripes);
-            else if (a.nditems > 0)
-                hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(blocks_for(a.nditems, kWavesPerBlock)),
+            const ItemGrid g = item_grid(a.nditems, a.nditems_dev != nullptr, a.n);
+            if (g.waves > 0)
+                hipLaunchKernelGGL((k_border_boxes<T, D>), dim3(blocks_for(g.waves, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, a, core, bcluster, a.inrow, labels, stats,
                                    amb_list, noise_stripes);
+            if (g.tail)
+                hipLaunchKernelGGL((k_border_boxes_walk<T, D>),
+                                   dim3(blocks_for(kTailWaves, kWavesPerBlock)), dim3(kThreads), 0,
+                                   s, a, core, bcluster, a.inrow, labels, stats, amb_list,
+                                   noise_stripes, static_cast<int32_t>(g.waves));
             if (a.nsitems > 0)
                 hipLaunchKernelGGL((k_border_sparse<T, D>), dim3(blocks_for(a.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, a, core, bcluster, labels, stats,

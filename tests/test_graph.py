@@ -152,3 +152,31 @@ def test_graph_batch_workload_sample(plain, graph):
     finally:
         xd.close()
         ld.close()
+
+
+@pytest.mark.parametrize("split", [(8, False), (0, False), (0, True)])
+def test_graph_sparse_rows_beyond_the_likely_grid(plain, graph, split):
+    """~1.5 rows per cell: more chunk items than the one-wave-per-item grid
+    of a device-written count covers (an eighth of the rows), so the
+    walking grid takes the rest (core count; border labels with every cell
+    dense)."""
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0, 100.0, (60_000, 2)).astype(np.float32)
+    say(f"graph sparse rows, work split {split}")
+    graph.set_work_split(*split)
+    want = oracle.cluster(x, 0.5, 3)
+    got = graph.cluster(x, 0.5, 3)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got, plain.cluster(x, 0.5, 3))
+    st = graph.graph_stats()
+    assert st["captures"] == 1 and st["fallbacks"] == 0, st
+    xd = _lib.DeviceBuffer.from_array(graph, x)
+    ld = _lib.DeviceBuffer(graph, 8 * len(x))
+    try:
+        for _ in range(2):
+            graph.cluster_device(xd.ptr, len(x), 2, np.float32, 0.5, 3, ld.ptr)
+            np.testing.assert_array_equal(ld.to_array(np.int64, len(x)), want)
+    finally:
+        xd.close()
+        ld.close()
