@@ -656,10 +656,19 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     tm.mark(M_UNION);
     const bool one_thread = ctx.border_rule == HPD_BORDER_ONE_THREAD;
     int2* amb = one_thread ? A.take_n<int2>(nn) : nullptr;
-    // box mode reads rootslot only under the one-thread border rule
+    // box mode reads rootslot only under the one-thread border rule; without
+    // it the rows of full boxes are labelled in input order, through a table
+    // indexed by cell key in the sort's spare key buffer when the keys fit
+    const bool full_rows = box_union && !one_thread;
+    unsigned int* full_ok = full_rows ? &counters[13] : nullptr;
+    if (full_rows)
+        launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, host_count(nbox),
+                                        bcluster, reinterpret_cast<int32_t*>(keys_free),
+                                        static_cast<int64_t>(nn * sizeof(K) / 4), labels,
+                                        &dstats[1], full_ok, s);
     launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin,
                           box_union && !one_thread ? nullptr : rootslot, cmin, cmax, labels,
-                          &dstats[1], s);
+                          &dstats[1], s, full_ok);
     tm.mark(M_BORDER0);
     launch_border<T, D>(a, core, box_union ? bcluster : nullptr, rootslot, cmin, cmax, labels,
                         &dstats[2], amb, noise_stripes, s);
@@ -743,6 +752,7 @@ enum StaticSlot : int {
     SS_NONFINITE = 14, // non-finite coordinates
     SS_GRID = 15,      // grid beyond 64 key bits
     SS_BITS = 16,      // key bits + sub-box bits
+    SS_ROWTAB = 17,    // the row-order label table holds every key
     SS_WORDS = 32
 };
 
@@ -972,8 +982,14 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     launch_box_clusters(NB, bmincore, bcell, cbox, bstart, inrow_of_slot, ustart, bparent,
                         reinterpret_cast<unsigned long long*>(scratch64), compmin, bcluster, s);
 
+    // the rows of full boxes labelled in input order when the keys fit the
+    // table (in the sort's spare key buffer), the other cores in slot order
+    launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, NB, bcluster,
+                                    reinterpret_cast<int32_t*>(keys_b),
+                                    static_cast<int64_t>(nn * sizeof(K) / 4), labels, &dstats[1],
+                                    &sc[SS_ROWTAB], s);
     launch_label_cores<T>(a, core, bcluster, parent, compmin, nullptr, cmin, cmax, labels,
-                          &dstats[1], s);
+                          &dstats[1], s, &sc[SS_ROWTAB]);
     launch_border<T, D>(a, core, bcluster, rootslot, cmin, cmax, labels, &dstats[2], nullptr,
                         noise_stripes, s);
 }

@@ -37,6 +37,7 @@
 #include <type_traits>
 
 #include "boxes.hpp"
+#include "cellkey.hpp"
 #include "common.hpp"
 #include "dbscan.hpp"
 #include "grid.hpp"
@@ -474,26 +475,81 @@ __global__ __launch_bounds__(kThreads) void k_label_cores(
 // 8 waves on each of the 1024 SIMDs
 constexpr unsigned kLabelBlocks = 2048;
 
+// skip_full (with *full_ok set): the rows of full boxes (btight bit 2) were
+// labelled in input order (k_label_full_rows) and are skipped here.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
     int64_t n, const JobParams<T>* __restrict__ jobs, const int32_t* __restrict__ job_of_cell,
     const uint8_t* __restrict__ core, const int32_t* __restrict__ box_of_slot,
     const int32_t* __restrict__ bcell, const int32_t* __restrict__ bcluster,
     const uint32_t* __restrict__ inrow, int64_t* __restrict__ labels,
-    int32_t* __restrict__ rootslot, unsigned long long* __restrict__ ncore, bool one_job) {
+    int32_t* __restrict__ rootslot, unsigned long long* __restrict__ ncore, bool one_job,
+    const uint8_t* __restrict__ skip_full, const unsigned int* __restrict__ full_ok) {
+    if (skip_full && *full_ok == 0) skip_full = nullptr;
     // grid-stride over at most kLabelBlocks blocks, so that the core count
     // of the run statistics (folded in here) takes few atomics
     unsigned long long cores = 0;
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; p < n;
          p += static_cast<int64_t>(gridDim.x) * kThreads) {
         if (!core[p]) continue;
-        ++cores;
         const int32_t b = box_of_slot[p];
+        if (skip_full && (skip_full[b] & 2)) continue;
+        ++cores;
         const int32_t r = bcluster[b];
         if (rootslot) rootslot[p] = r;
         // one job starts at row 0
         const int64_t first_pos = one_job ? 0 : jobs[job_of_cell[bcell[b]]].row_begin;
         labels[inrow[p]] = -(static_cast<int64_t>(r) - first_pos + 1);
+    }
+    if (ncore) block_add<kThreads>(cores, ncore);
+}
+
+// Row-order labels of full boxes.  A table indexed by cell key (the sort's
+// keys start at 0 and are dense enough when the largest one is below the
+// table's capacity) holds, for the key of every full box (tight with >=
+// minPts rows: every row core, cluster = the box's), the label's magnitude
+// bcluster - first row of the job + 1, and 0 for the keys of other boxes.
+// Keys no box has are never looked up.  *ok = the largest key fits.
+template <typename T, typename K>
+__global__ __launch_bounds__(kThreads) void k_full_box_table(
+    const K* __restrict__ keys, int64_t n, const int32_t* __restrict__ bstart, Count nbox,
+    const int32_t* __restrict__ bcell, const int32_t* __restrict__ job_of_cell,
+    const JobParams<T>* __restrict__ jobs, const uint8_t* __restrict__ btight,
+    const int32_t* __restrict__ bcluster, int32_t* __restrict__ tab, int64_t cap,
+    unsigned int* __restrict__ ok) {
+    const bool fits = static_cast<uint64_t>(keys[n - 1]) < static_cast<uint64_t>(cap);
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b == 0) *ok = fits ? 1u : 0u;
+    if (!fits || b >= nbox.get()) return;
+    int32_t v = 0;
+    if (btight[b] & 2)
+        v = static_cast<int32_t>(static_cast<int64_t>(bcluster[b]) -
+                                 jobs[job_of_cell[bcell[b]]].row_begin + 1);
+    tab[static_cast<int64_t>(keys[bstart[b]])] = v;
+}
+
+// The labels of the rows of full boxes, written in input order: each row's
+// key is computed again from its coordinates exactly as the sort's keys
+// were (row_key), looked up in the table, and the label stored at the row's
+// own place (coalesced, where labelling in slot order stores at permuted
+// places).  Rows of other boxes (table value 0) are left to
+// k_label_cores_boxes.  Nothing when the table did not fit.
+template <typename T, int D, typename K>
+__global__ __launch_bounds__(kThreads) void k_label_full_rows(
+    const T* __restrict__ pts, int64_t n, const JobParams<T>* __restrict__ jobs, int32_t njobs,
+    int sub_k, int sub_bits, const int32_t* __restrict__ tab, const unsigned int* __restrict__ ok,
+    int64_t* __restrict__ labels, unsigned long long* __restrict__ ncore) {
+    if (*ok == 0) return;
+    unsigned long long cores = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kThreads) {
+        const JobParams<T>& jp = jobs[njobs == 1 ? 0 : job_of_row(jobs, njobs, i)];
+        const K key = static_cast<K>(row_key<T, D>(pts + i * D, jp, sub_k, sub_bits));
+        const int32_t v = tab[static_cast<int64_t>(key)];
+        if (v != 0) {
+            labels[i] = -static_cast<int64_t>(v);
+            ++cores;
+        }
     }
     if (ncore) block_add<kThreads>(cores, ncore);
 }
@@ -1061,6 +1117,22 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
     HPD_HIP(hipGetLastError());
 }
 
+template <typename T, int D, typename K>
+void launch_label_full_rows(const T* pts, const DbscanArgs<T>& a, int sub_k, int sub_bits,
+                            const K* keys, Count nbox, const int32_t* bcluster, int32_t* tab,
+                            int64_t cap, int64_t* labels, unsigned long long* ncore,
+                            unsigned int* ok, hipStream_t s) {
+    if (a.n <= 0 || nbox.host <= 0) return;
+    hipLaunchKernelGGL((k_full_box_table<T, K>), dim3(blocks_for(nbox.host, kThreads)),
+                       dim3(kThreads), 0, s, keys, a.n, a.bstart, nbox, a.bcell, a.job_of_cell,
+                       a.jobs, a.btight, bcluster, tab, cap, ok);
+    const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
+    hipLaunchKernelGGL((k_label_full_rows<T, D, K>), dim3(grid), dim3(kThreads), 0, s, pts, a.n,
+                       a.jobs, a.njobs, sub_k, sub_bits, static_cast<const int32_t*>(tab),
+                       static_cast<const unsigned int*>(ok), labels, ncore);
+    HPD_HIP(hipGetLastError());
+}
+
 template <typename T, int D>
 void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* parent,
                          int32_t* rsnap, int32_t* cmin, int32_t* cmax, int32_t* compmin,
@@ -1086,13 +1158,14 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s) {
+                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s,
+                        const unsigned int* full_ok) {
     if (a.n <= 0) return;
     if (bcluster && a.bcell) {
         const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
         hipLaunchKernelGGL(k_label_cores_boxes<T>, dim3(grid), dim3(kThreads), 0, s, a.n, a.jobs,
                            a.job_of_cell, core, a.box_of_slot, a.bcell, bcluster, a.inrow, labels,
-                           rootslot, ncore, a.njobs == 1);
+                           rootslot, ncore, a.njobs == 1, full_ok ? a.btight : nullptr, full_ok);
         HPD_HIP(hipGetLastError());
         return;
     }
@@ -1206,7 +1279,33 @@ HPD_FOR_EACH_DTYPE(HPD_INST_DBSCAN_ALL_D)
 #define HPD_INST_LABEL_CORES(T)                                                                \
     template void launch_label_cores<T>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,  \
                                         int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, \
-                                        int64_t*, unsigned long long*, hipStream_t);
+                                        int64_t*, unsigned long long*, hipStream_t,      \
+                                        const unsigned int*);
 HPD_FOR_EACH_DTYPE(HPD_INST_LABEL_CORES)
+
+#define HPD_INST_FULL_ROWS(T, D, K)                                                             \
+    template void launch_label_full_rows<T, D, K>(const T*, const DbscanArgs<T>&, int, int,     \
+                                                  const K*, Count, const int32_t*, int32_t*,     \
+                                                  int64_t, int64_t*, unsigned long long*,        \
+                                                  unsigned int*, hipStream_t);
+#define HPD_INST_FULL_ROWS_ALL(T)          \
+    HPD_INST_FULL_ROWS(T, 1, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 2, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 3, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 4, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 5, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 6, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 7, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 8, uint32_t)     \
+    HPD_INST_FULL_ROWS(T, 1, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 2, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 3, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 4, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 5, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 6, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 7, uint64_t)     \
+    HPD_INST_FULL_ROWS(T, 8, uint64_t)
+HPD_INST_FULL_ROWS_ALL(float)
+HPD_INST_FULL_ROWS_ALL(double)
 
 }  // namespace hpd

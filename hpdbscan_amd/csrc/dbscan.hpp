@@ -86,10 +86,26 @@ void launch_union_points(const DbscanArgs<T>& a, const uint8_t* core, int32_t* p
 // Labels go to input order through a.inrow; *ncore += number of cores.
 // rootslot: ld entries (may be null in box mode, where only the one-thread
 // border rule reads it).
+// full_ok (device word, box mode): when *full_ok is set,
+// launch_label_full_rows labelled the rows of the full boxes, which are
+// skipped here.
 template <typename T>
 void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32_t* bcluster,
                         int32_t* parent, const int32_t* compmin, int32_t* rootslot, int32_t* cmin,
-                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s);
+                        int32_t* cmax, int64_t* labels, unsigned long long* ncore, hipStream_t s,
+                        const unsigned int* full_ok = nullptr);
+
+// Box mode (floating types): labels of the rows of full boxes (tight with
+// >= minPts rows, all core, cluster = bcluster of the box) written in input
+// order.  tab (cap int32 entries, scratch) is indexed by the sorted keys;
+// when the largest key is below cap, *ok = 1 and each row's label is looked
+// up by its key (recomputed from pts with the same sub-box split) and
+// *ncore += those rows; otherwise *ok = 0 and nothing is labelled.
+template <typename T, int D, typename K>
+void launch_label_full_rows(const T* pts, const DbscanArgs<T>& a, int sub_k, int sub_bits,
+                            const K* keys, Count nbox, const int32_t* bcluster, int32_t* tab,
+                            int64_t cap, int64_t* labels, unsigned long long* ncore,
+                            unsigned int* ok, hipStream_t s);
 
 // Border and noise labels of the non-core rows (in box mode box by box:
 // all cores of a box share its cluster).  stats[0] += noise rows,
