@@ -227,10 +227,12 @@ def batched_leg(a) -> dict:
             greps.append(time.perf_counter() - t)
         g_labels = gl.to_array(np.int64, len(rows))
         t_graph = float(np.median(greps))
+        small = _small_batch(ctx, gctx, rows, off, eps, mps, a.batch_reps)
         graph = {"graph_jobs_per_s": round(nj / t_graph, 1), "graph_s": round(t_graph, 4),
                  "graph_first_call_s": round(t_capture, 4),
                  "graph_labels_equal": bool(np.array_equal(g_labels, batched_labels)),
                  "graph_stats": gctx.graph_stats(),
+                 "graph_small": small,
                  "graph_note": "static pipeline (worst-case tables, device-side counts and grids, "
                                "no host read-back) captured in a hipGraph by the first call and "
                                "replayed"}
@@ -268,6 +270,35 @@ def batched_leg(a) -> dict:
         "note": "device-resident rows; a batched job's latency is the whole batch's",
         **graph,
     }
+
+
+def _small_batch(ctx, gctx, rows, off, eps, mps, reps, jobs=256) -> dict:
+    """The `jobs` smallest clouds of the batch as one batch, where the plain
+    pipeline's host read-backs weigh most: plain batched call vs the
+    hipGraph replay, median over 5 * reps calls each."""
+    pick = np.argsort(np.diff(off), kind="stable")[:jobs]
+    x, o = _lib.concat_jobs([rows[off[j]:off[j + 1]] for j in pick])
+    e, m = eps[pick], mps[pick]
+    out = {"jobs": int(len(pick)), "rows": int(len(x))}
+    labels = []
+    for name, c in (("plain_batched", ctx), ("graph", gctx)):
+        xd = _lib.DeviceBuffer.from_array(c, x)
+        ld = _lib.DeviceBuffer(c, 8 * len(x))
+        c.cluster_batch_device(xd.ptr, o, 3, np.float32, e, m, ld.ptr)
+        c.synchronize()
+        ts = []
+        for _ in range(5 * reps):
+            t = time.perf_counter()
+            c.cluster_batch_device(xd.ptr, o, 3, np.float32, e, m, ld.ptr)
+            c.synchronize()
+            ts.append(time.perf_counter() - t)
+        labels.append(ld.to_array(np.int64, len(x)))
+        out[f"{name}_ms"] = round(float(np.median(ts)) * 1e3, 3)
+        out[f"{name}_jobs_per_s"] = round(len(pick) / float(np.median(ts)), 1)
+        xd.close()
+        ld.close()
+    out["labels_equal"] = bool(np.array_equal(labels[0], labels[1]))
+    return out
 
 
 def serving_leg(a) -> dict:
