@@ -184,6 +184,12 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
                                                 int32_t wid) {
     Chunk c;
     chunk_at(a.ditems, wid, a.ustart, a.roff, c);
+    // the first group's neighbour boxes are loaded ahead, beside the rows
+    // and their boxes (the chunk was selected because a row needs counting)
+    const int32_t center = uniform(a.rcenter[c.u]);
+    const int32_t own_box = uniform(a.cbox[c.u]);
+    RowBoxes pre;
+    row_boxes(c.r0, c.r1, center, 0, a.rcells, a.cbox, INT_MAX, pre.rb0, pre.rcnt);
     const JobParams<T>& jp = a.jobs[uniform(a.job_of_cell[c.u])];
     const T eps2 = jp.eps2;
     const uint32_t min_pts = jp.min_pts;
@@ -209,7 +215,7 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
         double clo[D], chi[D];
         rows_bbox<T, D>(pc, need, clo, chi);
         for_each_row_box_group<T, D>(
-            c.r0, c.r1, uniform(a.rcenter[c.u]), a.rcells, a.cbox, a.bstart,
+            c.r0, c.r1, center, a.rcells, a.cbox, a.bstart,
             static_cast<const OrdKey<T>*>(a.bbox), nullptr,
             [&](int32_t nk, int32_t bl, const double* lol, const double* hil, int32_t q0l,
                 int32_t q1l, int32_t) {
@@ -223,7 +229,50 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
                 for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
                 if (need) cnt += add;
                 need = valid && cnt < min_pts;
-                for (uint64_t todo = ballot(some); todo != 0 && __any(need); todo &= todo - 1) {
+                const uint64_t somem = ballot(some);
+                const uint64_t needm = ballot(need);
+                if (D <= 4 && __popcll(needm) < __popcll(somem)) {
+                    // fewer rows than boxes to decide: row by row, every
+                    // box on its lane at once (the same point-box test as
+                    // box by box below); the boxes within eps of a row's
+                    // farthest corner count whole, the others that are
+                    // near it are marked for the row and tested row by row
+                    uint64_t part_rows = 0;  // on a box's lane: rows it is partial for
+                    for (uint64_t rows = needm; rows != 0; rows &= rows - 1) {
+                        const int r = static_cast<int>(__builtin_ctzll(rows));
+                        double v[D];
+#pragma unroll
+                        for (int e = 0; e < D; ++e) v[e] = bcast(static_cast<double>(pc[e]), r);
+                        double gmin2, gmax2;
+                        point_box_dd<D>(lol, hil, v, gmin2, gmax2);
+                        const bool near = some && gmin2 <= gap2_max;
+                        const bool whole = near && gmax2 <= full2;
+                        uint32_t w = whole ? static_cast<uint32_t>(q1l - q0l) : 0u;
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+                        if (lane_id() == r) cnt += w;
+                        if (near && !whole) part_rows |= uint64_t{1} << r;
+                    }
+                    need = valid && cnt < min_pts;
+                    for (uint64_t todo = ballot(part_rows != 0); todo != 0 && __any(need);
+                         todo &= todo - 1) {
+                        const int j = static_cast<int>(__builtin_ctzll(todo));
+                        const bool part = need && ((bcast64(part_rows, j) >> lane_id()) & 1);
+                        if (__any(part)) {
+                            auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
+                                cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
+                            };
+                            auto stop = [&] {
+                                return static_cast<bool>(__all(!part || cnt >= min_pts));
+                            };
+                            visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, bcast(q0l, j),
+                                              bcast(q1l, j), body, stop);
+                        }
+                        need = valid && cnt < min_pts;
+                    }
+                    return !__any(need);
+                }
+                for (uint64_t todo = somem; todo != 0 && __any(need); todo &= todo - 1) {
                     const int j = static_cast<int>(__builtin_ctzll(todo));
                     double lo[D], hi[D];
 #pragma unroll
@@ -250,7 +299,7 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
                 }
                 return !__any(need);
             },
-            INT_MAX, uniform(a.cbox[c.u]));
+            INT_MAX, own_box, &pre);
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;

@@ -122,6 +122,11 @@ __device__ __forceinline__ void point_box(const OrdKey<T>* __restrict__ bbox, in
 __device__ __forceinline__ int32_t bcast(int32_t v, int j) {
     return __builtin_amdgcn_readlane(v, j);
 }
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int j) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), j));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), j));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 __device__ __forceinline__ double bcast(double v, int j) {
     const long long x = __double_as_longlong(v);
     const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), j));
@@ -132,19 +137,26 @@ __device__ __forceinline__ double bcast(double v, int j) {
 
 // Squared distances from a row to the box [lo, hi] (double): gmin2 to its
 // nearest point, gmax2 to its farthest corner.
-template <typename T, int D>
-__device__ __forceinline__ void point_box_lh(const double* lo, const double* hi, const T* pc,
+template <int D>
+__device__ __forceinline__ void point_box_dd(const double* lo, const double* hi, const double* v,
                                              double& gmin2, double& gmax2) {
     gmin2 = 0.0;
     gmax2 = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        const double v = static_cast<double>(pc[j]);
-        const double g = fmax(0.0, fmax(lo[j] - v, v - hi[j]));
-        const double f = fmax(v - lo[j], hi[j] - v);
+        const double g = fmax(0.0, fmax(lo[j] - v[j], v[j] - hi[j]));
+        const double f = fmax(v[j] - lo[j], hi[j] - v[j]);
         gmin2 += g * g;
         gmax2 += f * f;
     }
+}
+template <typename T, int D>
+__device__ __forceinline__ void point_box_lh(const double* lo, const double* hi, const T* pc,
+                                             double& gmin2, double& gmax2) {
+    double v[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = static_cast<double>(pc[j]);
+    point_box_dd<D>(lo, hi, v, gmin2, gmax2);
 }
 
 // Visit the boxes of the neighbour rows [r0, r1) of an occupied cell, the
@@ -229,6 +241,29 @@ __device__ __forceinline__ void for_each_row_box(int32_t r0, int32_t r1, int32_t
 // order of for_each_row_box (lane order inside a group).  Lets a kernel
 // classify a whole group against the bounding box of its rows in parallel
 // and walk only the boxes that need row tests.
+// The boxes of the neighbour rows g0 + lane of the visiting order (own row
+// first): the first box rb0 and the number rcnt below bmax (0 past the rows).
+__device__ __forceinline__ void row_boxes(int32_t r0, int32_t r1, int32_t center, int32_t g0,
+                                          const int2* __restrict__ rcells,
+                                          const int32_t* __restrict__ cbox, int32_t bmax,
+                                          int32_t& rb0, int32_t& rcnt) {
+    const int32_t i = g0 + __lane_id();
+    rb0 = 0;
+    rcnt = 0;
+    if (i < r1 - r0) {
+        const int32_t r = i == 0 ? center : (r0 + i - 1 < center ? r0 + i - 1 : r0 + i);
+        const int2 cs = rcells[r];
+        rb0 = cbox[cs.x];
+        rcnt = max(0, min(cbox[cs.y], bmax) - rb0);
+    }
+}
+
+// pre (optional): row_boxes of the first group, loaded by the caller ahead
+// of its own loads so that the two overlap.
+struct RowBoxes {
+    int32_t rb0, rcnt;
+};
+
 template <typename T, int D, typename Fn>
 __device__ __forceinline__ void for_each_row_box_group(int32_t r0, int32_t r1, int32_t center,
                                                        const int2* __restrict__ rcells,
@@ -237,17 +272,17 @@ __device__ __forceinline__ void for_each_row_box_group(int32_t r0, int32_t r1, i
                                                        const OrdKey<T>* __restrict__ bbox,
                                                        const int32_t* __restrict__ tag, Fn&& fn,
                                                        int32_t bmax = INT_MAX,
-                                                       int32_t first = -1) {
+                                                       int32_t first = -1,
+                                                       const RowBoxes* pre = nullptr) {
     const int32_t nrows = r1 - r0;
     const int lane = __lane_id();
     for (int32_t g0 = 0; g0 < nrows; g0 += 64) {
-        const int32_t i = g0 + lane;  // visiting order: own row, then the others
-        int32_t rb0 = 0, rcnt = 0;
-        if (i < nrows) {
-            const int32_t r = i == 0 ? center : (r0 + i - 1 < center ? r0 + i - 1 : r0 + i);
-            const int2 cs = rcells[r];
-            rb0 = cbox[cs.x];
-            rcnt = max(0, min(cbox[cs.y], bmax) - rb0);  // boxes below bmax only
+        int32_t rb0, rcnt;
+        if (g0 == 0 && pre) {
+            rb0 = pre->rb0;
+            rcnt = pre->rcnt;
+        } else {
+            row_boxes(r0, r1, center, g0, rcells, cbox, bmax, rb0, rcnt);
         }
         int32_t incl = rcnt;  // inclusive prefix over the lanes
 #pragma unroll
