@@ -102,8 +102,12 @@ def measured_traffic(workload: str, kernel: str):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel)
-    return int(k["bytes_per_launch"]) if k else None
+    # entries are keyed by the kernel's name with its template arguments;
+    # "a+b" sums the kernels a and b
+    want = set(kernel.split("+"))
+    hits = [int(v["bytes_per_launch"]) for k, v in t.get("kernels", {}).items()
+            if k.split("<")[0] in want]
+    return sum(hits) if hits else None
 
 
 def kernel_roofline(st: dict, med: dict, n: int, d: int, s: int, workload: str) -> dict:
@@ -449,8 +453,12 @@ def p1_leg(a) -> dict:
         out["cpu"] = {k: v for k, v in ref.items() if k != "labels"}
         out["cpu_one_thread"] = {k: v for k, v in ref1.items() if k != "labels"}
         if ref1.get("labels") is not None:
-            # at one thread the reference's labels are fixed: all equal
-            out["parity_one_thread_ref"] = parity_fields(labels, ref1["labels"])
+            # at one thread the reference's labels are fixed: under the
+            # one-thread border rule every label is equal; under the default
+            # rule only border rows next to two clusters may differ
+            out["parity_one_thread_ref"] = parity_fields(_one_thread_labels(x, eps, mp),
+                                                         ref1["labels"])
+            out["parity_default_rule"] = parity_fields(labels, ref1["labels"])
         if ref.get("value"):
             out["speedup_vs_cpu"] = round(v / ref["value"], 1)
     return out
