@@ -1013,7 +1013,7 @@ void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose
 // count costs a dispatch.
 constexpr int64_t kMaxItemWaves = int64_t{1} << 24;
 inline unsigned wave_blocks(Count items) {
-    int64_t w = items.dev ? std::min(items.host, items.host / 8 + 64) : items.host;
+    int64_t w = items.dev && !items.tight ? std::min(items.host, items.host / 8 + 64) : items.host;
     w = std::min(w, kMaxItemWaves);
     return blocks_for(w > 0 ? w : 1, kWavesPerBlock);
 }

@@ -404,9 +404,6 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // occupied cells (spatial_index.h:167-182); their order and the order of
     // input rows inside them are the reference's sorted order
     launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &counters[0], s);
-    HPD_HIP(hipMemcpyAsync(hc, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HPD_HIP(hipStreamSynchronize(s));
-    const int32_t m = static_cast<int32_t>(hc[0]);
 
     // box mode: slots are rows sorted by cell, sub-box, input row; the
     // reference's sorted positions are only materialised where needed
@@ -441,6 +438,11 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     } else {
         launch_gather_soa<T, D>(pts, n, inrow_of_slot, soa, ld, s);
     }
+    // one read-back for the occupied cells and the boxes
+    HPD_HIP(hipMemcpyAsync(hc, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipStreamSynchronize(s));
+    const int32_t m = static_cast<int32_t>(hc[0]);
+    const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
     tm.mark(M_SORT);
 
     // neighbour ranges (spatial_index.h:682-737) and work items
@@ -474,6 +476,17 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     }
     exclusive_scan_set_u32(counts, ncounts, m, tmp, tmp_words, s);
     if (boxes) {
+        // the box chunks (counted and scanned) and the tight flags (their
+        // loose count decides the union) before the read-back below
+        launch_box_items(bstart, host_count(nbox), bcnt, tmp, &counters[4], s);
+        BoxArgs<T> bt;
+        bt.nbox = nbox;
+        bt.bstart = bstart;
+        bt.bcell = bcell;
+        bt.bbox = bbox;
+        bt.jobs = jobs;
+        bt.job_of_cell = job_of_cell;
+        launch_box_tight<T, D>(bt, btight, &counters[5], s);
         HPD_HIP(hipMemcpyAsync(&counters[11], &ccnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         HPD_HIP(hipMemcpyAsync(&counters[6], &dcnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         HPD_HIP(hipMemcpyAsync(&counters[7], &scnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
@@ -490,16 +503,11 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     const uint32_t total_ranges = hc[1];
     const int32_t ncitems = boxes ? static_cast<int32_t>(hc[11]) : 0;
     const int32_t nitems = static_cast<int32_t>(hc[2]);
-    const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
     const int32_t nditems = boxes ? static_cast<int32_t>(hc[6]) : 0;
     const int32_t nsitems = boxes ? static_cast<int32_t>(hc[7]) : 0;
-    int32_t nbitems = 0;
-    if (boxes) {
-        launch_box_items(bstart, host_count(nbox), bcnt, tmp, &counters[4], s);
-        HPD_HIP(hipMemcpyAsync(hc + 4, &counters[4], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HPD_HIP(hipStreamSynchronize(s));
-        nbitems = static_cast<int32_t>(hc[4]);
-    }
+    const int32_t nbitems = boxes ? static_cast<int32_t>(hc[4]) : 0;
+    // every box of two or more rows is tight: the union runs on boxes
+    const bool all_tight = boxes && hc[5] == 0;
 
     X.reserve(2 * align_up(static_cast<size_t>(total_ranges) * sizeof(int2), 256) +
                     align_up(static_cast<size_t>(nitems) * sizeof(int2), 256) +
@@ -565,7 +573,6 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         ba.jobs = jobs;
         ba.job_of_cell = job_of_cell;
         ba.rcenter = rcenter;
-        launch_box_tight<T, D>(ba, btight, &counters[5], s);
         a.box_of_slot = box_of_slot;
         a.btight = btight;
         a.cbox = cbox;
@@ -590,28 +597,22 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     tm.mark(M_CORE);
 
     // box lists of the union (box mode leaves the slot-level parent free)
+    // (the lists' lengths stay on the device: boxes with cores, and the
+    // roots of each join round, bounded by the number of boxes)
     int32_t* blist = parent;
-    int32_t nlist = 0;
-    if (boxes) {
-        launch_select_boxes(host_count(nbox), 0, bmincore, blist, &counters[8], s);
-        HPD_HIP(hipMemcpyAsync(hc + 5, &counters[5], 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HPD_HIP(hipStreamSynchronize(s));
-        box_union = hc[5] == 0;  // every box of two or more rows is tight
-        nlist = static_cast<int32_t>(hc[8]);  // boxes with cores
-    }
+    box_union = all_tight;
     if (box_union) {
         // compmin holds the forest's roots until the clusters are known
         tm.mark(M_HOOK0);
-        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, host_count(nlist),
-                              bparent, s);
+        launch_select_boxes(host_count(nbox), 0, bmincore, blist, &counters[8], s);
+        launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist,
+                              Count{nbox, &counters[8], true}, bparent, s);
         launch_box_roots(host_count(nbox), bmincore, bparent, compmin, s);
         for (int round = 0; round < kRootJoinRounds<D>; ++round) {
             HPD_HIP(hipMemsetAsync(&counters[9], 0, sizeof(uint32_t), s));
             launch_select_boxes(host_count(nbox), 1, compmin, blist, &counters[9], s);
-            HPD_HIP(hipMemcpyAsync(hc + 9, &counters[9], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            HPD_HIP(hipStreamSynchronize(s));
             launch_box_root_join<T, D>(ba, nranges, rcells, compmin, core, blist,
-                                       host_count(hc[9]), bparent, s);
+                                       Count{nbox, &counters[9]}, bparent, s);
             launch_box_roots(host_count(nbox), bmincore, bparent, compmin, s);
         }
         tm.mark(M_HOOK1);

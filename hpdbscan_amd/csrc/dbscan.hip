@@ -586,15 +586,19 @@ __global__ __launch_bounds__(kThreads) void k_full_box_table(
 template <typename T, int D, typename K>
 __global__ __launch_bounds__(kThreads) void k_label_full_rows(
     const T* __restrict__ pts, int64_t n, const JobParams<T>* __restrict__ jobs, int32_t njobs,
-    int sub_k, int sub_bits, const int32_t* __restrict__ tab, const unsigned int* __restrict__ ok,
-    int64_t* __restrict__ labels, unsigned long long* __restrict__ ncore) {
+    int sub_k, int sub_bits, const int32_t* __restrict__ tab, int64_t cap,
+    const unsigned int* __restrict__ ok, int64_t* __restrict__ labels,
+    unsigned long long* __restrict__ ncore) {
     if (*ok == 0) return;
     unsigned long long cores = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kThreads) {
         const JobParams<T>& jp = jobs[njobs == 1 ? 0 : job_of_row(jobs, njobs, i)];
         const K key = static_cast<K>(row_key<T, D>(pts + i * D, jp, sub_k, sub_bits));
-        const int32_t v = tab[static_cast<int64_t>(key)];
+        // (every row's key is a box's key, below cap when *ok is set)
+        const int32_t v = static_cast<uint64_t>(key) < static_cast<uint64_t>(cap)
+                              ? tab[static_cast<int64_t>(key)]
+                              : 0;
         if (v != 0) {
             labels[i] = -static_cast<int64_t>(v);
             ++cores;
@@ -1177,7 +1181,7 @@ void launch_label_full_rows(const T* pts, const DbscanArgs<T>& a, int sub_k, int
                        a.jobs, a.btight, bcluster, tab, cap, ok);
     const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
     hipLaunchKernelGGL((k_label_full_rows<T, D, K>), dim3(grid), dim3(kThreads), 0, s, pts, a.n,
-                       a.jobs, a.njobs, sub_k, sub_bits, static_cast<const int32_t*>(tab),
+                       a.jobs, a.njobs, sub_k, sub_bits, static_cast<const int32_t*>(tab), cap,
                        static_cast<const unsigned int*>(ok), labels, ncore);
     HPD_HIP(hipGetLastError());
 }

@@ -64,10 +64,14 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // A number of items known on the host, or (dev not null) written into device
 // memory by an earlier kernel, `host` being then an upper bound that sizes
-// the launch.  The static pipeline (no host read-backs) passes the latter.
+// the launch (tight: a bound close to the count, such as the number of boxes
+// for a list of boxes, rather than a worst case).  The static pipeline (no
+// host read-backs) passes the latter throughout, the plain one for lists
+// whose length it does not wait for.
 struct Count {
     int64_t host = 0;
     const uint32_t* dev = nullptr;
+    bool tight = false;  // host is close to the count: one wave per item of it
     __device__ __forceinline__ int64_t get() const {
         return dev ? static_cast<int64_t>(*dev) : host;
     }
