@@ -111,6 +111,7 @@ SIGNATURES = {
     "hpd_context_destroy": (ctypes.c_int, [_vp]),
     "hpd_context_set_border_rule": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hpd_context_set_work_split": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int]),
+    "hpd_context_set_open_pair_limit": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "hpd_context_get_work_split": (ctypes.c_int, [_vp, _vp, _vp]),
     "hpd_context_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hpd_context_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -221,6 +222,11 @@ class Context:
         """Cells with at most sparse_max rows go one thread per row (border
         pass; core count too with sparse_core).  Results do not change."""
         check(load().hpd_context_set_work_split(self._h, int(sparse_max), 1 if sparse_core else 0))
+
+    def set_open_pair_limit(self, limit: int) -> None:
+        """Most open box pairs the union lists hold (0: 2n); beyond it the
+        union walks every box's neighbourhood.  Results do not change."""
+        check(load().hpd_context_set_open_pair_limit(self._h, int(limit)))
 
     def work_split(self) -> tuple[int, bool]:
         sm, sc = ctypes.c_int32(), ctypes.c_int()

@@ -753,18 +753,16 @@ __global__ __launch_bounds__(kThreads) void k_box_roots(Count nbox_c,
 // than eps; otherwise the lanes whose row is a core within eps of the other
 // box's bounding box test the other box's cores as they stream through the
 // wave, until the first within-eps pair, which joins the two boxes.
+// One chunk of k_box_union.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_box_union(
-    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+__device__ __forceinline__ void box_union_item(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
     const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
     const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
     const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
     const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
-    const int32_t* __restrict__ rsnap, int32_t* bparent) {
-    const int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
-                                static_cast<int32_t>(threadIdx.x >> 6));
-    if (wid >= nitems) return;
+    const int32_t* __restrict__ rsnap, int32_t* bparent, int32_t wid) {
     const int2 it = bitems[wid];
     const int32_t b = uniform(it.x);
     if (bmincore[b] == INT_MAX) return;
@@ -825,6 +823,29 @@ __global__ __launch_bounds__(kThreads) void k_box_union(
             }
         }
     }
+}
+
+// The full union walk: every box chunk against the later boxes of its
+// neighbourhood.  The fallback when the open-pair lists do not fit, so it is
+// launched on a small grid whose waves walk the chunks (only_over: run only
+// when *only_over > cap, decided on the device).
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_union(
+    const T* __restrict__ soa, int64_t ld, const int2* __restrict__ bitems, int32_t nitems,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcell,
+    const int32_t* __restrict__ cbox, const uint32_t* __restrict__ roff,
+    const int2* __restrict__ rcells, const OrdKey<T>* __restrict__ bbox,
+    const int32_t* __restrict__ bmincore, const uint8_t* __restrict__ core,
+    const int32_t* __restrict__ job_of_cell, const JobParams<T>* __restrict__ jobs,
+    const int32_t* __restrict__ rsnap, int32_t* bparent,
+    const unsigned long long* __restrict__ only_over, unsigned long long cap) {
+    if (only_over && *only_over <= cap) return;
+    const int32_t stride = static_cast<int32_t>(gridDim.x) * kWavesPerBlock;
+    for (int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
+                               static_cast<int32_t>(threadIdx.x >> 6));
+         wid < nitems; wid += stride)
+        box_union_item<T, D>(soa, ld, bitems, bstart, bcell, cbox, roff, rcells, bbox, bmincore,
+                             core, job_of_cell, jobs, rsnap, bparent, wid);
 }
 
 // Cluster ids.  A sorted position orders rows by (occupied cell, input
@@ -1090,12 +1111,15 @@ void launch_select_items(Count nitems, const int2* items, const uint32_t* off, i
 template <typename T, int D>
 void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
-                     int32_t* bparent, hipStream_t s) {
+                     int32_t* bparent, hipStream_t s, const unsigned long long* only_over,
+                     unsigned long long cap) {
     using U = OrdKey<T>;
-    hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(a.nbitems, kWavesPerBlock)),
+    if (a.nbitems <= 0) return;
+    const int64_t waves = std::min<int64_t>(a.nbitems, 32768);
+    hipLaunchKernelGGL((k_box_union<T, D>), dim3(blocks_for(waves, kWavesPerBlock)),
                        dim3(kThreads), 0, s, a.soa, a.ld, a.bitems, a.nbitems, a.bstart, a.bcell,
                        a.cbox, roff, rcells, reinterpret_cast<const U*>(a.bbox), bmincore, core,
-                       a.job_of_cell, a.jobs, rsnap, bparent);
+                       a.job_of_cell, a.jobs, rsnap, bparent, only_over, cap);
 }
 
 template <typename T, int D>
@@ -1195,7 +1219,8 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
                                              Count, int32_t*, hipStream_t);                       \
     template void launch_box_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, const uint8_t*, const int32_t*, int32_t*, \
-                                        hipStream_t);                                             \
+                                        hipStream_t, const unsigned long long*,                   \
+                                        unsigned long long);                                      \
     template void launch_box_scan<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, uint32_t*, int32_t*,                      \
                                         unsigned long long*, hipStream_t,                         \

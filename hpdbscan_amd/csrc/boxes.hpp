@@ -103,10 +103,13 @@ void launch_select_core_chunks(Count nitems, const int2* items, const int32_t* u
                                unsigned int* count, hipStream_t s);
 void launch_select_items(Count nitems, const int2* items, const uint32_t* off, int2* out,
                          unsigned int* count, hipStream_t s);
+// only_over (optional): the join runs only when *only_over > cap, i.e. when
+// the open-pair lists did not fit (decided on the device, no host check).
 template <typename T, int D>
 void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* rsnap,
-                     int32_t* bparent, hipStream_t s);
+                     int32_t* bparent, hipStream_t s, const unsigned long long* only_over = nullptr,
+                     unsigned long long cap = 0);
 
 // The pairs the hook forest leaves open.  launch_box_scan with list ==
 // nullptr writes cnt[b] (nbox entries) and adds their sum to *total (64-bit,
@@ -114,8 +117,9 @@ void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
 // list[cnt[b] ..] (cnt scanned to offsets in between).
 // launch_box_union_list joins the listed pairs that hold a within-eps core
 // pair; off has nbox + 1 entries.  When the list does not fit, the caller
-// runs launch_box_join instead.  With total_chk (the static pipeline, no
-// host check in between), both do nothing when *total_chk > cap.
+// runs launch_box_join instead.  With total_chk (no host check in between),
+// both do nothing when *total_chk > cap, and launch_box_join with the same
+// check runs exactly then.
 template <typename T, int D>
 void launch_box_scan(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* rsnap, uint32_t* cnt, int32_t* list,

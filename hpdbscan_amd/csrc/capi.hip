@@ -223,6 +223,7 @@ struct Context {
     // the others one wave per 64-row chunk (hpd_context_set_work_split;
     // defaults picked by hpdbscan_amd.autotune on the 10M x 3-D workload)
     int32_t sparse_max = 8;
+    uint64_t open_pair_limit = 0;  // 0: 2n
     bool sparse_core = false;
     bool timing = false;
     hpd_run_stats stats{};
@@ -619,25 +620,20 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         // the open pairs of every box, as a list in scratch64; offsets in
         // bcnt (free after the box items)
         int32_t* list = reinterpret_cast<int32_t*>(scratch64);
-        const uint64_t cap = 2 * static_cast<uint64_t>(nn);
+        unsigned long long cap = 2 * static_cast<unsigned long long>(nn);
+        if (ctx.open_pair_limit > 0 && ctx.open_pair_limit < cap) cap = ctx.open_pair_limit;
         launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
-        // offsets (meaningful when the total fits, checked below) and the
-        // chunks of the boxes that have open pairs
+        // offsets (meaningful when the total fits; the kernels that use them
+        // check the 64-bit total on the device, and the full join runs
+        // instead when it does not fit) and the chunks of the boxes that
+        // have open pairs
         exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
         launch_select_items(host_count(nbitems), bitems, bcnt, uitems, &counters[10], s);
-        unsigned long long* hs64 = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
-        HPD_HIP(hipMemcpyAsync(hs64 + 8, &dstats[5], sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, s));
-        HPD_HIP(hipMemcpyAsync(hc + 20, &counters[10], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HPD_HIP(hipStreamSynchronize(s));
-        ctx.stats.open_box_pairs = hs64[8];
-        if (hs64[8] <= cap) {  // then the 32-bit offsets cannot overflow
-            launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s);
-            launch_box_union_list<T, D>(ba, uitems, host_count(hc[20]), core, bcnt, list, bparent,
-                                        s);
-        } else {
-            launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s);
-        }
+        launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s, &dstats[5], cap);
+        launch_box_union_list<T, D>(ba, uitems, Count{nbitems, &counters[10]}, core, bcnt, list,
+                                    bparent, s, &dstats[5], cap);
+        launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s, &dstats[5],
+                              cap);
         tm.mark(M_JOIN1);
         launch_box_clusters(host_count(nbox), bmincore, bcell, cbox, bstart, inrow_of_slot, ustart,
                             bparent, reinterpret_cast<unsigned long long*>(scratch64), compmin,
@@ -675,8 +671,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                         &dstats[2], amb, noise_stripes, s);
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
-    HPD_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hs, dstats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
+    if (box_union) ctx.stats.open_box_pairs = hs[5];
     if (one_thread && hs[3] > 0) {
         // the reference's choice at one thread for the border rows next to
         // two or more clusters (hpdbscan.h:74-92)
@@ -1508,6 +1505,13 @@ int hpd_context_get_graph_stats(const hpd_context* ctx, uint64_t* launches, uint
         *launches = ctx->impl.graph_launches;
         *captures = ctx->impl.graph_captures;
         *fallbacks = ctx->impl.graph_fallbacks;
+    });
+}
+
+int hpd_context_set_open_pair_limit(hpd_context* ctx, uint64_t limit) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        ctx->impl.open_pair_limit = limit;
     });
 }
 

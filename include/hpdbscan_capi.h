@@ -143,6 +143,12 @@ int hpd_context_get_graph_stats(const hpd_context* ctx, uint64_t* launches, uint
 int hpd_context_set_work_split(hpd_context* ctx, int32_t sparse_max, int sparse_core);
 int hpd_context_get_work_split(const hpd_context* ctx, int32_t* sparse_max, int* sparse_core);
 
+/* Most box pairs the union keeps in its open-pair lists (0: the default,
+ * 2n for n rows, which the lists' memory holds).  When a run has more, the
+ * union walks every box's neighbourhood instead (decided on the device).
+ * Results do not depend on it.  No counterpart in the reference. */
+int hpd_context_set_open_pair_limit(hpd_context* ctx, uint64_t limit);
+
 /*
  * Cluster n rows of d values of type `dtype` (row-major, host memory).
  * labels_out (host, n entries) receives the reference's labels in input

@@ -183,6 +183,28 @@ def test_box_union_used_and_exact(ctx):
     assert_parity(x, 0.3, 8, got, oracle.cluster(x, 0.3, 8))
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_open_pair_limit_takes_the_full_join(ctx, d):
+    """With the open-pair lists limited to one pair, the union walks every
+    box's neighbourhood instead (decided on the device): same labels as with
+    the lists, and the oracle's."""
+    from hpdbscan_amd import _lib
+    x = blobs(30000, d, 20, 0.15, {2: 6.0, 3: 8.0}[d], 40 + d, np.float32)
+    eps, mp = {2: (0.1, 8), 3: (0.3, 8)}[d]
+    say(f"open-pair limit 1, d={d}")
+    want = ctx.cluster(x, eps, mp)
+    assert ctx.stats()["open_box_pairs"] > 1
+    c = _lib.Context(0)
+    try:
+        c.set_open_pair_limit(1)
+        got = c.cluster(x, eps, mp)
+        assert c.stats()["box_union"] == 1
+    finally:
+        c.close()
+    np.testing.assert_array_equal(got, want)
+    assert_parity(x, eps, mp, got, oracle.cluster(x, eps, mp))
+
+
 def dense_blobs(n, d, seed):
     """Four tight blobs on separated centres with a little noise around them:
     few occupied cells, each holding many rows."""
