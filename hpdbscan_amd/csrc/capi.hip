@@ -351,8 +351,8 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
 
 template <typename T, int D, typename K>
 static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>* jobs,
-                      int32_t njobs, int key_bits, bool boxes, int64_t* labels, hipStream_t s,
-                      PhaseTimer& tm) {
+                      int32_t njobs, int key_bits, uint64_t key_total, bool boxes,
+                      int64_t* labels, hipStream_t s, PhaseTimer& tm) {
     Arena& A = *ctx.main_use;
     Arena& X = *ctx.aux_use;
     const size_t nn = static_cast<size_t>(n);
@@ -444,6 +444,14 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     HPD_HIP(hipStreamSynchronize(s));
     const int32_t m = static_cast<int32_t>(hc[0]);
     const int32_t nbox = boxes ? static_cast<int32_t>(hc[3]) : 0;
+    // box mode: a dense table of every cell id in the sort's spare key
+    // buffer, when the grid is small enough (the neighbour rows then read
+    // the table instead of searching the occupied cells)
+    int32_t* ctab = nullptr;
+    if (boxes && key_total > 0 && key_total <= nn * sizeof(K) / sizeof(int32_t)) {
+        ctab = reinterpret_cast<int32_t*>(keys_free);
+        launch_cell_table<K>(ucell, host_count(m), ctab, key_total, s);
+    }
     tm.mark(M_SORT);
 
     // neighbour ranges (spatial_index.h:682-737) and work items
@@ -458,7 +466,8 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // up to 27 rows per cell (D <= 4): the copy puts the nearest rows first
     const bool by_class = stash && D <= 4;
     launch_neighbor_count<K, T>(ucell, ustart, host_count(m), jobs, njobs, job_of_cell, nranges, ccnt,
-                                &dstats[0], &dstats[4], stash, stash_rows, rcenter, by_class, s);
+                                &dstats[0], &dstats[4], stash, stash_rows, rcenter, by_class, s,
+                                ctab);
     launch_chunk_counts(ustart, host_count(m), nchunks, s);
     // box mode: dense-cell chunks and sparse-cell rows as separate work
     // lists; their per-cell counts live in the (now unused) run flags
@@ -531,7 +540,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                              by_class, s);
     else
         launch_neighbor_fill<K, T>(ucell, ustart, host_count(m), jobs, job_of_cell, nranges, ranges,
-                                   rcells, rcenter, s);
+                                   rcells, rcenter, s, ctab);
     launch_work_items(ustart, host_count(m), nchunks, host_count(nitems), items, s);
     if (boxes) {
         launch_split_items(ustart, host_count(m), ctx.sparse_max, dcnt, scnt, ditems,
@@ -656,13 +665,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // box mode reads rootslot only under the one-thread border rule; without
     // it the rows of full boxes are labelled in input order, through a table
     // indexed by cell key in the sort's spare key buffer when the keys fit
-    const bool full_rows = box_union && !one_thread;
+    // (launched only when every key fits the table; *full_ok stays 0 else)
+    const bool full_rows = box_union && !one_thread &&
+                           key_total <= (nn * sizeof(K) / sizeof(int32_t)) >> sub_bits;
     unsigned int* full_ok = full_rows ? &counters[13] : nullptr;
     if (full_rows)
         launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, host_count(nbox),
                                         bcluster, reinterpret_cast<int32_t*>(keys_free),
                                         static_cast<int64_t>(nn * sizeof(K) / 4), labels,
-                                        &dstats[1], full_ok, s);
+                                        full_ok, s);
     launch_label_cores<T>(a, core, box_union ? bcluster : nullptr, parent, compmin,
                           box_union && !one_thread ? nullptr : rootslot, cmin, cmax, labels,
                           &dstats[1], s, full_ok);
@@ -984,7 +995,7 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     // table (in the sort's spare key buffer), the other cores in slot order
     launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, NB, bcluster,
                                     reinterpret_cast<int32_t*>(keys_b),
-                                    static_cast<int64_t>(nn * sizeof(K) / 4), labels, &dstats[1],
+                                    static_cast<int64_t>(nn * sizeof(K) / 4), labels,
                                     &sc[SS_ROWTAB], s);
     launch_label_cores<T>(a, core, bcluster, parent, compmin, nullptr, cmin, cmax, labels,
                           &dstats[1], s, &sc[SS_ROWTAB]);
@@ -1213,9 +1224,11 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
     const bool boxes = std::is_floating_point<T>::value && key_bits + box_sub_bits(D) <= 64;
     const int sort_bits = key_bits + (boxes ? box_sub_bits(D) : 0);
     if (sort_bits <= 32)
-        run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, boxes, labels, s, tm);
+        run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, key_total, boxes, labels,
+                                  s, tm);
     else
-        run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, boxes, labels, s, tm);
+        run_keyed<T, D, uint64_t>(ctx, pts, n, jobs_dev, njobs, key_bits, key_total, boxes, labels,
+                                  s, tm);
     if (ctx.timing) {
         ctx.stats.ms_grid = tm.ms(M_START, M_GRID);
         ctx.stats.ms_sort = tm.ms(M_GRID, M_SORT);

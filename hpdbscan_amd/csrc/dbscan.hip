@@ -541,9 +541,9 @@ __global__ __launch_bounds__(kThreads) void k_label_cores_boxes(
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; p < n;
          p += static_cast<int64_t>(gridDim.x) * kThreads) {
         if (!core[p]) continue;
+        ++cores;  // every core, the row-order ones included
         const int32_t b = box_of_slot[p];
         if (skip_full && (skip_full[b] & 2)) continue;
-        ++cores;
         const int32_t r = bcluster[b];
         if (rootslot) rootslot[p] = r;
         // one job starts at row 0
@@ -587,10 +587,8 @@ template <typename T, int D, typename K>
 __global__ __launch_bounds__(kThreads) void k_label_full_rows(
     const T* __restrict__ pts, int64_t n, const JobParams<T>* __restrict__ jobs, int32_t njobs,
     int sub_k, int sub_bits, const int32_t* __restrict__ tab, int64_t cap,
-    const unsigned int* __restrict__ ok, int64_t* __restrict__ labels,
-    unsigned long long* __restrict__ ncore) {
+    const unsigned int* __restrict__ ok, int64_t* __restrict__ labels) {
     if (*ok == 0) return;
-    unsigned long long cores = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kThreads) {
         const JobParams<T>& jp = jobs[njobs == 1 ? 0 : job_of_row(jobs, njobs, i)];
@@ -599,12 +597,8 @@ __global__ __launch_bounds__(kThreads) void k_label_full_rows(
         const int32_t v = static_cast<uint64_t>(key) < static_cast<uint64_t>(cap)
                               ? tab[static_cast<int64_t>(key)]
                               : 0;
-        if (v != 0) {
-            labels[i] = -static_cast<int64_t>(v);
-            ++cores;
-        }
+        if (v != 0) labels[i] = -static_cast<int64_t>(v);
     }
-    if (ncore) block_add<kThreads>(cores, ncore);
 }
 
 template <typename T, int D>
@@ -1173,16 +1167,15 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
 template <typename T, int D, typename K>
 void launch_label_full_rows(const T* pts, const DbscanArgs<T>& a, int sub_k, int sub_bits,
                             const K* keys, Count nbox, const int32_t* bcluster, int32_t* tab,
-                            int64_t cap, int64_t* labels, unsigned long long* ncore,
-                            unsigned int* ok, hipStream_t s) {
+                            int64_t cap, int64_t* labels, unsigned int* ok, hipStream_t s) {
     if (a.n <= 0 || nbox.host <= 0) return;
     hipLaunchKernelGGL((k_full_box_table<T, K>), dim3(blocks_for(nbox.host, kThreads)),
                        dim3(kThreads), 0, s, keys, a.n, a.bstart, nbox, a.bcell, a.job_of_cell,
                        a.jobs, a.btight, bcluster, tab, cap, ok);
-    const unsigned grid = std::min<unsigned>(blocks_for(a.n, kThreads), kLabelBlocks);
-    hipLaunchKernelGGL((k_label_full_rows<T, D, K>), dim3(grid), dim3(kThreads), 0, s, pts, a.n,
+    hipLaunchKernelGGL((k_label_full_rows<T, D, K>), dim3(blocks_for(a.n, kThreads)),
+                       dim3(kThreads), 0, s, pts, a.n,
                        a.jobs, a.njobs, sub_k, sub_bits, static_cast<const int32_t*>(tab), cap,
-                       static_cast<const unsigned int*>(ok), labels, ncore);
+                       static_cast<const unsigned int*>(ok), labels);
     HPD_HIP(hipGetLastError());
 }
 
@@ -1339,8 +1332,7 @@ HPD_FOR_EACH_DTYPE(HPD_INST_LABEL_CORES)
 #define HPD_INST_FULL_ROWS(T, D, K)                                                             \
     template void launch_label_full_rows<T, D, K>(const T*, const DbscanArgs<T>&, int, int,     \
                                                   const K*, Count, const int32_t*, int32_t*,     \
-                                                  int64_t, int64_t*, unsigned long long*,        \
-                                                  unsigned int*, hipStream_t);
+                                                  int64_t, int64_t*, unsigned int*, hipStream_t);
 #define HPD_INST_FULL_ROWS_ALL(T)          \
     HPD_INST_FULL_ROWS(T, 1, uint32_t)     \
     HPD_INST_FULL_ROWS(T, 2, uint32_t)     \
@@ -1360,5 +1352,6 @@ HPD_FOR_EACH_DTYPE(HPD_INST_LABEL_CORES)
     HPD_INST_FULL_ROWS(T, 8, uint64_t)
 HPD_INST_FULL_ROWS_ALL(float)
 HPD_INST_FULL_ROWS_ALL(double)
+
 
 }  // namespace hpd

@@ -99,13 +99,13 @@ void launch_label_cores(const DbscanArgs<T>& a, const uint8_t* core, const int32
 // >= minPts rows, all core, cluster = bcluster of the box) written in input
 // order.  tab (cap int32 entries, scratch) is indexed by the sorted keys;
 // when the largest key is below cap, *ok = 1 and each row's label is looked
-// up by its key (recomputed from pts with the same sub-box split) and
-// *ncore += those rows; otherwise *ok = 0 and nothing is labelled.
+// up by its key (recomputed from pts with the same sub-box split);
+// otherwise *ok = 0 and nothing is labelled.  launch_label_cores counts
+// every core, these rows included.
 template <typename T, int D, typename K>
 void launch_label_full_rows(const T* pts, const DbscanArgs<T>& a, int sub_k, int sub_bits,
                             const K* keys, Count nbox, const int32_t* bcluster, int32_t* tab,
-                            int64_t cap, int64_t* labels, unsigned long long* ncore,
-                            unsigned int* ok, hipStream_t s);
+                            int64_t cap, int64_t* labels, unsigned int* ok, hipStream_t s);
 
 // Border and noise labels of the non-core rows (in box mode box by box:
 // all cores of a box share its cluster).  stats[0] += noise rows,

@@ -16,7 +16,10 @@
 // table.  A cell's ranges are stored in ascending row order, or (D <= 4 with
 // the count pass's stash) nearest first: the own row, then the rows offset in
 // one, two, three dimensions.  The index of the row holding the cell itself
-// is kept separately.
+// is kept separately.  When the grid is small enough for a dense table of
+// every cell id (ctab: occupied-cell index or -1), a row's cells are three
+// table reads instead of a binary search.
+#include "boxes.hpp"
 #include "common.hpp"
 #include "grid.hpp"
 #include "index.hpp"
@@ -76,6 +79,29 @@ __device__ __forceinline__ int2 row_cells(const K* __restrict__ ucell, int32_t m
     int32_t b = a;
     while (b < m && static_cast<uint64_t>(ucell[b]) <= hi) ++b;
     return make_int2(a, b);
+}
+
+// The same from the dense cell table: the occupied ones among the (at most
+// three) ids lo..hi, which are consecutive in the occupied-cell order.
+__device__ __forceinline__ int2 row_cells_tab(const int32_t* __restrict__ ctab, uint64_t lo,
+                                              uint64_t hi) {
+    int32_t a = INT32_MAX, b = -1;
+    for (uint64_t c = lo; c <= hi; ++c) {
+        const int32_t u = ctab[c];
+        if (u >= 0) {
+            a = min(a, u);
+            b = max(b, u);
+        }
+    }
+    return b >= 0 ? make_int2(a, b + 1) : make_int2(0, 0);
+}
+
+// ctab[id] = u for every occupied cell u (the table filled with -1 before)
+template <typename K>
+__global__ __launch_bounds__(256) void k_cell_table(const K* __restrict__ ucell, Count mc,
+                                                    int32_t* __restrict__ ctab) {
+    const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (u < static_cast<int32_t>(mc.get())) ctab[static_cast<uint64_t>(ucell[u])] = u;
 }
 
 // Decode a cell id into per-swapped-dimension indices.
@@ -156,7 +182,8 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
                                                              int2* __restrict__ stash,
                                                              int32_t stash_rows,
                                                              int32_t* __restrict__ center_local,
-                                                             bool by_class) {
+                                                             bool by_class,
+                                                             const int32_t* __restrict__ ctab) {
     const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     uint64_t cand = 0;
@@ -171,8 +198,9 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_count(const K* __restrict
         int32_t from = 0, center = 0;
         int2* mine = stash ? stash + static_cast<int64_t>(u) * stash_rows : nullptr;
         for_each_row(key - base, jobs[j].cp, [&](uint64_t lo, uint64_t hi, bool is_center, int nz) {
-            const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
-            from = ab.y;
+            const int2 ab = ctab ? row_cells_tab(ctab, base + lo, base + hi)
+                                 : row_cells(ucell, m, base + lo, base + hi, from);
+            if (!ctab) from = ab.y;
             if (ab.y > ab.x) {
                 if (mine) {
                     if (is_center) center = static_cast<int32_t>(cnt);
@@ -216,7 +244,8 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
                                                             const uint32_t* __restrict__ roff,
                                                             int2* __restrict__ ranges,
                                                             int2* __restrict__ rcells,
-                                                            int32_t* __restrict__ rcenter) {
+                                                            int32_t* __restrict__ rcenter,
+                                                            const int32_t* __restrict__ ctab) {
     const int32_t m = static_cast<int32_t>(mc.get());
     const int32_t u = static_cast<int32_t>(blockIdx.x) * 256 + threadIdx.x;
     if (u >= m) return;
@@ -227,8 +256,9 @@ __global__ __launch_bounds__(256) void k_neighbor_rows_fill(const K* __restrict_
     int32_t from = 0;
     for_each_row(static_cast<uint64_t>(ucell[u]) - base, jobs[j].cp,
                  [&](uint64_t lo, uint64_t hi, bool is_center, int) {
-                     const int2 ab = row_cells(ucell, m, base + lo, base + hi, from);
-                     from = ab.y;
+                     const int2 ab = ctab ? row_cells_tab(ctab, base + lo, base + hi)
+                                          : row_cells(ucell, m, base + lo, base + hi, from);
+                     if (!ctab) from = ab.y;
                      if (ab.y > ab.x) {
                          if (is_center) center = static_cast<int32_t>(w);
                          rcells[w] = ab;
@@ -431,11 +461,20 @@ void launch_neighbor_count(const K* ucell, const int32_t* ustart, Count m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
                            unsigned long long* rows_total, int2* stash, int32_t stash_rows,
-                           int32_t* rcenter, bool by_class, hipStream_t s) {
+                           int32_t* rcenter, bool by_class, hipStream_t s, const int32_t* ctab) {
     if (m.host <= 0) return;
     hipLaunchKernelGGL((k_neighbor_rows_count<K, T>), dim3(blocks_for(m.host, 256)), dim3(256), 0, s,
                        ucell, ustart, m, jobs, njobs, job_of_cell, nranges, core_chunks, sumcand,
-                       rows_total, stash, stash_rows, rcenter, by_class);
+                       rows_total, stash, stash_rows, rcenter, by_class, ctab);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename K>
+void launch_cell_table(const K* ucell, Count m, int32_t* ctab, uint64_t ids, hipStream_t s) {
+    if (m.host <= 0) return;
+    fill_i32(ctab, -1, static_cast<int64_t>(ids), s);
+    hipLaunchKernelGGL(k_cell_table<K>, dim3(blocks_for(m.host, 256)), dim3(256), 0, s, ucell, m,
+                       ctab);
     HPD_HIP(hipGetLastError());
 }
 
@@ -459,10 +498,10 @@ template <typename K, typename T>
 void launch_neighbor_fill(const K* ucell, const int32_t* ustart, Count m,
                           const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
-                          hipStream_t s) {
+                          hipStream_t s, const int32_t* ctab) {
     if (m.host <= 0) return;
     hipLaunchKernelGGL((k_neighbor_rows_fill<K, T>), dim3(blocks_for(m.host, 256)), dim3(256), 0, s,
-                       ucell, ustart, m, jobs, job_of_cell, roff, ranges, rcells, rcenter);
+                       ucell, ustart, m, jobs, job_of_cell, roff, ranges, rcells, rcenter, ctab);
     HPD_HIP(hipGetLastError());
 }
 
@@ -480,7 +519,8 @@ void launch_work_items(const int32_t* ustart, Count m, const uint32_t* coff, Cou
 
 #define HPD_INST_INDEX(K)                                                                     \
     template void launch_cells<K>(const K*, int64_t, int, uint32_t*, uint32_t*, K*, int32_t*, \
-                                  uint32_t*, hipStream_t);
+                                  uint32_t*, hipStream_t);                                    \
+    template void launch_cell_table<K>(const K*, Count, int32_t*, uint64_t, hipStream_t);
 HPD_INST_INDEX(uint32_t)
 HPD_INST_INDEX(uint64_t)
 
@@ -489,20 +529,22 @@ HPD_INST_INDEX(uint64_t)
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
                                                      unsigned long long*, int2*, int32_t,       \
-                                                     int32_t*, bool, hipStream_t);              \
+                                                     int32_t*, bool, hipStream_t,               \
+                                                     const int32_t*);                           \
     template void launch_neighbor_count<uint64_t, T>(const uint64_t*, const int32_t*, Count,    \
                                                      const JobParams<T>*, int32_t, int32_t*,    \
                                                      uint32_t*, uint32_t*, unsigned long long*, \
                                                      unsigned long long*, int2*, int32_t,       \
-                                                     int32_t*, bool, hipStream_t);              \
+                                                     int32_t*, bool, hipStream_t,               \
+                                                     const int32_t*);                           \
     template void launch_neighbor_fill<uint32_t, T>(const uint32_t*, const int32_t*, Count,     \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \
-                                                    hipStream_t);                               \
+                                                    hipStream_t, const int32_t*);               \
     template void launch_neighbor_fill<uint64_t, T>(const uint64_t*, const int32_t*, Count,     \
                                                     const JobParams<T>*, const int32_t*,        \
                                                     const uint32_t*, int2*, int2*, int32_t*,    \
-                                                    hipStream_t);
+                                                    hipStream_t, const int32_t*);
 HPD_FOR_EACH_DTYPE(HPD_INST_ROWS)
 
 }  // namespace hpd

@@ -31,7 +31,14 @@ void launch_neighbor_count(const K* ucell, const int32_t* ustart, Count m,
                            const JobParams<T>* jobs, int32_t njobs, int32_t* job_of_cell,
                            uint32_t* nranges, uint32_t* core_chunks, unsigned long long* sumcand,
                            unsigned long long* rows_total, int2* stash, int32_t stash_rows,
-                           int32_t* rcenter, bool by_class, hipStream_t s);
+                           int32_t* rcenter, bool by_class, hipStream_t s,
+                           const int32_t* ctab = nullptr);
+
+// Dense table of every cell id below `ids`: ctab[id] = its occupied-cell
+// index, -1 for an empty cell (ids int32 entries).  With it the neighbour
+// kernels read a row's cells from the table instead of searching.
+template <typename K>
+void launch_cell_table(const K* ucell, Count m, int32_t* ctab, uint64_t ids, hipStream_t s);
 
 // launch_neighbor_fill's output from the count pass's stash (roff scanned);
 // with by_class the rows of each cell go nearest first (own row, then by
@@ -47,7 +54,7 @@ template <typename K, typename T>
 void launch_neighbor_fill(const K* ucell, const int32_t* ustart, Count m,
                           const JobParams<T>* jobs, const int32_t* job_of_cell,
                           const uint32_t* roff, int2* ranges, int2* rcells, int32_t* rcenter,
-                          hipStream_t s);
+                          hipStream_t s, const int32_t* ctab = nullptr);
 
 void launch_chunk_counts(const int32_t* ustart, Count m, uint32_t* nchunks, hipStream_t s);
 
