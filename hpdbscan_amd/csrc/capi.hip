@@ -497,16 +497,23 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         bt.jobs = jobs;
         bt.job_of_cell = job_of_cell;
         launch_box_tight<T, D>(bt, btight, &counters[5], s);
-        HPD_HIP(hipMemcpyAsync(&counters[11], &ccnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        HPD_HIP(hipMemcpyAsync(&counters[6], &dcnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        HPD_HIP(hipMemcpyAsync(&counters[7], &scnt[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     }
-    HPD_HIP(hipMemcpyAsync(&counters[1], &nranges[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    HPD_HIP(hipMemcpyAsync(&counters[2], &nchunks[m], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    HPD_HIP(hipMemcpyAsync(hc, counters, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    // the scans' totals into the counters (one launch), then one read-back
+    // of the 64-bit row total (dstats[4]) and the counters (dstats + 8)
+    WordCopies totals;
+    totals.add(&nranges[m], &counters[1]);
+    totals.add(&nchunks[m], &counters[2]);
+    if (boxes) {
+        totals.add(&ccnt[m], &counters[11]);
+        totals.add(&dcnt[m], &counters[6]);
+        totals.add(&scnt[m], &counters[7]);
+    }
+    copy_words(totals, s);
     unsigned long long* hs_rows = reinterpret_cast<unsigned long long*>(ctx.host_scratch) + 12;
-    HPD_HIP(hipMemcpyAsync(hs_rows, &dstats[4], sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hs_rows, &dstats[4], 10 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
+    hc = reinterpret_cast<uint32_t*>(hs_rows + 4);  // the counters as read back
     // the scan of nranges and every range index are 32-bit: check the
     // 64-bit total first
     check_neighbor_rows(*hs_rows);

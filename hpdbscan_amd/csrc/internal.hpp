@@ -91,6 +91,17 @@ void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, Count n, uint32_t
 // *dst = a[*idx_dev] (a scan's total at a device-written length, moved to a
 // fixed place)
 void copy_at_count(const uint32_t* a, const uint32_t* idx_dev, uint32_t* dst, hipStream_t s);
+// *dst[i] = *src[i] for i < k (device words, one launch)
+struct WordCopies {
+    const uint32_t* src[8];
+    uint32_t* dst[8];
+    int k = 0;
+    void add(const uint32_t* from, uint32_t* to) {
+        src[k] = from;
+        dst[k++] = to;
+    }
+};
+void copy_words(const WordCopies& c, hipStream_t s);
 
 // In-place exclusive scans of k <= kMaxScanSet arrays of one length n, each
 // array's total to its entry n (every array holds n + 1 entries), in two

@@ -244,6 +244,18 @@ void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, Count n, uint32_t
     HPD_HIP(hipGetLastError());
 }
 
+namespace {
+__global__ void k_copy_words(const WordCopies c) {
+    if (static_cast<int>(threadIdx.x) < c.k) *c.dst[threadIdx.x] = *c.src[threadIdx.x];
+}
+}  // namespace
+
+void copy_words(const WordCopies& c, hipStream_t s) {
+    if (c.k <= 0) return;
+    hipLaunchKernelGGL(k_copy_words, dim3(1), dim3(64), 0, s, c);
+    HPD_HIP(hipGetLastError());
+}
+
 void copy_at_count(const uint32_t* a, const uint32_t* idx_dev, uint32_t* dst, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_at_count, dim3(1), dim3(64), 0, s, a, idx_dev, dst);
     HPD_HIP(hipGetLastError());
