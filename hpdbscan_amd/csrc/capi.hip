@@ -358,9 +358,11 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     const size_t nn = static_cast<size_t>(n);
     const int64_t ld = (n + 15) & ~int64_t(15);  // rows are read in aligned batches of <= 16
     const size_t L = static_cast<size_t>(ld);
+    // each key buffer is followed by its value buffer, so that the pair the
+    // sort leaves free is one block (the dense cell table's room)
     K* keys_a = A.take_n<K>(nn);
-    K* keys_b = A.take_n<K>(nn);
     uint32_t* vals_a = A.take_n<uint32_t>(nn);
+    K* keys_b = A.take_n<K>(nn);
     uint32_t* vals_b = A.take_n<uint32_t>(nn);
     const size_t hist_words = radix_hist_words(n);
     uint32_t* hist = A.take_n<uint32_t>(hist_words);
@@ -448,8 +450,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // buffer, when the grid is small enough (the neighbour rows then read
     // the table instead of searching the occupied cells)
     int32_t* ctab = nullptr;
-    if (boxes && key_total > 0 && key_total <= nn * sizeof(K) / sizeof(int32_t)) {
-        ctab = reinterpret_cast<int32_t*>(keys_free);
+    const size_t free_pair = align_up(nn * sizeof(K), 256) + nn * sizeof(uint32_t);
+    if (boxes && key_total > 0 && key_total <= free_pair / sizeof(int32_t)) {
+        ctab = reinterpret_cast<int32_t*>(keys_free);  // keys_free and vals_free
         launch_cell_table<K>(ucell, host_count(m), ctab, key_total, s);
     }
     tm.mark(M_SORT);
