@@ -591,7 +591,7 @@ __global__ __launch_bounds__(kThreads) void k_label_full_rows(
     if (*ok == 0) return;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kThreads) {
-        const JobParams<T>& jp = jobs[njobs == 1 ? 0 : job_of_row(jobs, njobs, i)];
+        const JobParams<T>& jp = jobs[job_of_row_wave(jobs, njobs, i)];
         const K key = static_cast<K>(row_key<T, D>(pts + i * D, jp, sub_k, sub_bits));
         // (every row's key is a box's key, below cap when *ok is set)
         const int32_t v = static_cast<uint64_t>(key) < static_cast<uint64_t>(cap)

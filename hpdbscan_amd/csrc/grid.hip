@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_cell_ids(const T* __restrict__ pts, int
                                                   uint32_t* __restrict__ vals) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
-    const JobParams<T>& jp = jobs[job_of_row(jobs, njobs, i)];
+    const JobParams<T>& jp = jobs[job_of_row_wave(jobs, njobs, i)];
     keys[i] = static_cast<K>(row_key<T, D>(pts + i * D, jp, sub_k, sub_bits));
     vals[i] = static_cast<uint32_t>(i);
 }

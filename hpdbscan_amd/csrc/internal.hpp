@@ -183,4 +183,22 @@ __device__ __forceinline__ int32_t job_of_row(const JobParams<T>* __restrict__ j
     return lo;
 }
 
+// The same for the rows of a wave that are consecutive and ascending over
+// its lanes: one search for the first active lane's row (wave-uniform,
+// scalar loads), then each lane steps past the jobs that end before its
+// row (jobs hold many rows, so rarely more than one step).
+template <typename T>
+__device__ __forceinline__ int32_t job_of_row_wave(const JobParams<T>* __restrict__ jobs,
+                                                   int32_t njobs, int64_t row) {
+    if (njobs == 1) return 0;
+    const int64_t row0 = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(
+                             static_cast<int>(row & 0xffffffff))) |
+                         (static_cast<int64_t>(__builtin_amdgcn_readfirstlane(
+                              static_cast<int>(row >> 32)))
+                          << 32);
+    int32_t j = job_of_row(jobs, njobs, row0);
+    while (j + 1 < njobs && jobs[j].row_end <= row) ++j;
+    return j;
+}
+
 }  // namespace hpd
