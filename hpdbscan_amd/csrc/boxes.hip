@@ -46,6 +46,7 @@ constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kSmallBox = 32;
 constexpr int kSmallTests = 512;
 
+
 inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
 
 // Rows in slot order: coordinates (SoA); with it the bounding box of every
@@ -532,6 +533,64 @@ __global__ __launch_bounds__(kThreads) void k_box_union_list(
     }();
 }
 
+// Union over the open pairs of the boxes of at most small_rows rows, one
+// thread per box (a wave per box would leave most of its lanes idle): the
+// box's core rows meet each listed box whose root differs -- a row within
+// eps of the other box's whole bounding box joins at once, a row farther
+// than eps from it skips it, and otherwise the rows are tested against the
+// other box's cores until the first within-eps pair.  Every pair is decided
+// exactly (no test budget), as in k_box_union_list.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_union_small(
+    const T* __restrict__ soa, int64_t ld, Count nbox_c, const int32_t* __restrict__ bstart,
+    const int32_t* __restrict__ bcell, const OrdKey<T>* __restrict__ bbox,
+    const uint8_t* __restrict__ core, const int32_t* __restrict__ job_of_cell,
+    const JobParams<T>* __restrict__ jobs, const uint32_t* __restrict__ off,
+    const int32_t* __restrict__ list, int32_t* bparent,
+    const unsigned long long* __restrict__ total_chk, unsigned long long cap, int32_t small_rows) {
+    if (total_chk && *total_chk > cap) return;  // no list was written
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= static_cast<int32_t>(nbox_c.get())) return;
+    const uint32_t l0 = off[b], l1 = off[b + 1];
+    if (l0 == l1) return;
+    const int32_t s0 = bstart[b], s1 = bstart[b + 1];
+    if (s1 - s0 > small_rows) return;  // one wave per chunk instead
+    const JobParams<T>& jp = jobs[job_of_cell[bcell[b]]];
+    const T eps2 = jp.eps2;
+    const double gap2_max = jp.gap2_max;
+    const double full2 = static_cast<double>(eps2) * (1.0 - 1e-5);
+    for (uint32_t L = l0; L < l1; ++L) {
+        const int32_t b2 = list[L];
+        int32_t r = uf_find(bparent, b);
+        const int32_t r2 = uf_find(bparent, b2);
+        if (r == r2) continue;
+        const int32_t t0 = bstart[b2], t1 = bstart[b2 + 1];
+        bool hit = false;
+        for (int32_t p = s0; p < s1 && !hit; ++p) {
+            if (!core[p]) continue;
+            T pc[D];
+            load_row<T, D>(soa, ld, p, pc);
+            double gmin2, gmax2;
+            point_box<T, D>(bbox, b2, pc, gmin2, gmax2);
+            if (gmin2 > gap2_max) continue;
+            if (gmax2 <= full2) {  // within eps of all of b2, its cores included
+                hit = true;
+                break;
+            }
+            for (int32_t q = t0; q < t1; ++q) {
+                if (!core[q]) continue;
+                T qc[D];
+                load_row<T, D>(soa, ld, q, qc);
+                if (within<D>(pc, qc, eps2)) {
+                    hit = true;
+                    break;
+                }
+            }
+        }
+        if (hit) uf_unite(bparent, r, r2);
+    }
+}
+
 // Hook of a small box (at most kSmallBox rows), one thread per box: the
 // box's core rows meet the earlier boxes of its neighbourhood, own neighbour
 // row first; a box pair gets at most kSmallTests distance tests (rows
@@ -688,6 +747,8 @@ __global__ __launch_bounds__(kThreads) void k_select_boxes(Count nbox_c, int mod
 __global__ __launch_bounds__(kThreads) void k_select_items(Count nitems_c,
                                                            const int2* __restrict__ items,
                                                            const uint32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ bstart,
+                                                           int32_t min_rows,
                                                            int2* __restrict__ out,
                                                            unsigned int* __restrict__ count) {
     const int32_t nitems = static_cast<int32_t>(nitems_c.get());
@@ -695,7 +756,7 @@ __global__ __launch_bounds__(kThreads) void k_select_items(Count nitems_c,
         nitems, count,
         [&](int64_t i) {
             const int32_t b = items[i].x;
-            return off[b + 1] > off[b];
+            return off[b + 1] > off[b] && (!bstart || bstart[b + 1] - bstart[b] > min_rows);
         },
         [&](int64_t i, unsigned int pos) { out[pos] = items[i]; });
 }
@@ -1102,10 +1163,16 @@ void launch_select_core_chunks(Count nitems, const int2* items, const int32_t* u
 }
 
 void launch_select_items(Count nitems, const int2* items, const uint32_t* off, int2* out,
-                         unsigned int* count, hipStream_t s) {
+                         unsigned int* count, hipStream_t s, const int32_t* bstart,
+                         int32_t min_rows) {
     if (nitems.host <= 0) return;
     hipLaunchKernelGGL(k_select_items, dim3(blocks_for(nitems.host, kSelectSpan)), dim3(kThreads),
-                       0, s, nitems, items, off, out, count);
+                       0, s, nitems, items, off, min_rows > 0 ? bstart : nullptr, min_rows, out,
+                       count);
+}
+
+int32_t small_union_rows(int64_t n, int64_t nbox) {
+    return nbox > 0 && n < kSmallUnionMean * nbox ? kSmallUnionRows : 0;
 }
 
 template <typename T, int D>
@@ -1139,8 +1206,13 @@ template <typename T, int D>
 void launch_box_union_list(const BoxArgs<T>& a, const int2* items, Count nitems,
                            const uint8_t* core, const uint32_t* off, const int32_t* list,
                            int32_t* bparent, hipStream_t s, const unsigned long long* total_chk,
-                           unsigned long long cap) {
+                           unsigned long long cap, int32_t small_rows) {
     using U = OrdKey<T>;
+    if (a.nbox > 0 && small_rows > 0)
+        hipLaunchKernelGGL((k_box_union_small<T, D>), dim3(blocks_for(a.nbox, kThreads)),
+                           dim3(kThreads), 0, s, a.soa, a.ld, Count{a.nbox, a.nbox_dev}, a.bstart,
+                           a.bcell, reinterpret_cast<const U*>(a.bbox), core, a.job_of_cell,
+                           a.jobs, off, list, bparent, total_chk, cap, small_rows);
     if (nitems.host <= 0) return;
     hipLaunchKernelGGL((k_box_union_list<T, D>), dim3(wave_blocks(nitems)), dim3(kThreads), 0, s,
                        a.soa, a.ld, items, nitems, a.bstart, a.bcell,
@@ -1228,7 +1300,7 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
     template void launch_box_union_list<T, D>(const BoxArgs<T>&, const int2*, Count,               \
                                               const uint8_t*, const uint32_t*, const int32_t*,    \
                                               int32_t*, hipStream_t, const unsigned long long*,   \
-                                              unsigned long long);
+                                              unsigned long long, int32_t);
 #define HPD_INST_BOX_ALL_D(T) \
     HPD_INST_BOX_KEYS(T, 1)   \
     HPD_INST_BOX_KEYS(T, 2)   \

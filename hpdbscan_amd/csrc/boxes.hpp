@@ -101,8 +101,19 @@ void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btig
 void launch_select_core_chunks(Count nitems, const int2* items, const int32_t* ustart,
                                const int32_t* box_of_slot, const uint8_t* btight, int2* out,
                                unsigned int* count, hipStream_t s);
+// (with min_rows > 0: only chunks of boxes of more than min_rows rows; the
+// thread-per-box union of launch_box_union_list takes the others)
 void launch_select_items(Count nitems, const int2* items, const uint32_t* off, int2* out,
-                         unsigned int* count, hipStream_t s);
+                         unsigned int* count, hipStream_t s, const int32_t* bstart = nullptr,
+                         int32_t min_rows = 0);
+// Boxes of at most this many rows take their open pairs one thread per box
+// (0: none): on when the boxes hold fewer than kSmallUnionMean rows on
+// average (sparse data, where a wave per box leaves most lanes idle), off
+// for denser data, whose small boxes meet large ones that a wave tests
+// faster.
+constexpr int kSmallUnionRows = 8;
+constexpr int kSmallUnionMean = 8;
+int32_t small_union_rows(int64_t n, int64_t nbox);
 // only_over (optional): the join runs only when *only_over > cap, i.e. when
 // the open-pair lists did not fit (decided on the device, no host check).
 template <typename T, int D>
@@ -116,7 +127,8 @@ void launch_box_join(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcel
 // zeroed by the caller); with a list it writes every box's open partners to
 // list[cnt[b] ..] (cnt scanned to offsets in between).
 // launch_box_union_list joins the listed pairs that hold a within-eps core
-// pair; off has nbox + 1 entries.  When the list does not fit, the caller
+// pair (boxes of at most small_rows rows one thread per box, the items --
+// chunks of the other boxes -- one wave each); off has nbox + 1 entries.  When the list does not fit, the caller
 // runs launch_box_join instead.  With total_chk (no host check in between),
 // both do nothing when *total_chk > cap, and launch_box_join with the same
 // check runs exactly then.
@@ -130,7 +142,7 @@ void launch_box_union_list(const BoxArgs<T>& a, const int2* items, Count nitems,
                            const uint8_t* core, const uint32_t* off, const int32_t* list,
                            int32_t* bparent, hipStream_t s,
                            const unsigned long long* total_chk = nullptr,
-                           unsigned long long cap = 0);
+                           unsigned long long cap = 0, int32_t small_rows = 0);
 // p[0, n) = v
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 

@@ -647,10 +647,13 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         // instead when it does not fit) and the chunks of the boxes that
         // have open pairs
         exclusive_scan_u32(bcnt, bcnt, nbox, tmp, &bcnt[nbox], s);
-        launch_select_items(host_count(nbitems), bitems, bcnt, uitems, &counters[10], s);
+        // sparse data: the small boxes' pairs one thread per box
+        const int32_t small_rows = small_union_rows(n, nbox);
+        launch_select_items(host_count(nbitems), bitems, bcnt, uitems, &counters[10], s, bstart,
+                            small_rows);
         launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s, &dstats[5], cap);
         launch_box_union_list<T, D>(ba, uitems, Count{nbitems, &counters[10]}, core, bcnt, list,
-                                    bparent, s, &dstats[5], cap);
+                                    bparent, s, &dstats[5], cap, small_rows);
         launch_box_join<T, D>(ba, nranges, rcells, bmincore, core, compmin, bparent, s, &dstats[5],
                               cap);
         tm.mark(M_JOIN1);
@@ -994,10 +997,14 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     const unsigned long long cap = 2 * static_cast<unsigned long long>(nn);
     launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, nullptr, &dstats[5], s);
     exclusive_scan_u32_dev(bcnt, bcnt, NB, tmp, s);
-    launch_select_items(Count{IB, &sc[SS_BITEMS]}, bitems, bcnt, uitems, &sc[SS_UITEMS], s);
+    // (the box count is on the device: the small boxes' pairs always one
+    // thread per box, the static pipeline's batches of small clouds being
+    // sparse)
+    launch_select_items(Count{IB, &sc[SS_BITEMS]}, bitems, bcnt, uitems, &sc[SS_UITEMS], s, bstart,
+                        kSmallUnionRows);
     launch_box_scan<T, D>(ba, nranges, rcells, compmin, bcnt, list, nullptr, s, &dstats[5], cap);
     launch_box_union_list<T, D>(ba, uitems, Count{IB, &sc[SS_UITEMS]}, core, bcnt, list, bparent, s,
-                                &dstats[5], cap);
+                                &dstats[5], cap, kSmallUnionRows);
     launch_box_clusters(NB, bmincore, bcell, cbox, bstart, inrow_of_slot, ustart, bparent,
                         reinterpret_cast<unsigned long long*>(scratch64), compmin, bcluster, s);
 
