@@ -57,7 +57,8 @@ inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n +
 // when it goes on past the block, leave the block's share in
 // part[(2 * block + 0 / 1) * 2D ..] for k_box_span.  No global atomics: a
 // box of a million rows would otherwise take thousands of them on one
-// address.
+// address.  With pts null the coordinates are already in soa (the sort
+// moved them) and only the bounding boxes are made.
 template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_gather_boxes(
     const T* __restrict__ pts, int64_t n, const uint32_t* __restrict__ inrow_of_slot,
@@ -77,7 +78,11 @@ __global__ __launch_bounds__(kThreads) void k_gather_boxes(
     const int32_t b_first = box_of_slot[blk0];  // the block's first box
     int32_t b = -1 - lane_id();  // distinct keys for lanes past the end
     T x[D];
-    if (valid) {
+    if (valid && !pts) {
+        b = box_of_slot[s];
+#pragma unroll
+        for (int j = 0; j < D; ++j) x[j] = soa[static_cast<int64_t>(j) * ld + s];
+    } else if (valid) {
         const uint32_t row = inrow_of_slot[s];
         b = box_of_slot[s];
 #pragma unroll

@@ -30,9 +30,10 @@ struct BoxArgs {
     const int32_t* rcenter = nullptr;  // index of every cell's own neighbour row
 };
 
-// coordinates in slot order (soa[j * ld + s] = pts[inrow_of_slot[s] * D + j])
-// and the bounding box of every box (bbox, 2 * D ordered keys per box, all
-// written); part holds gather_part_bytes(n, D, sizeof(T)) bytes of scratch
+// coordinates in slot order (soa[j * ld + s] = pts[inrow_of_slot[s] * D + j];
+// with pts null they are there already) and the bounding box of every box
+// (bbox, 2 * D ordered keys per box, all written); part holds
+// gather_part_bytes(n, D, sizeof(T)) bytes of scratch
 template <typename T, int D>
 void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot, T* soa,
                          int64_t ld, const int32_t* box_of_slot, const int32_t* bstart, void* bbox,

@@ -339,13 +339,14 @@ static size_t workspace_bytes(int64_t n, int64_t njobs, bool one_thread) {
     b += 8 * nn + 2 * 4 * nn + 3 * 4 * (nn + 1) + 5 * 4 * nn + nn;  // scratch64, positions, box tables
     b += 2 * D * 8 * nn;                       // box bounding boxes
     b += gather_part_bytes(n, D, static_cast<int>(sizeof(T)));  // their per-block shares
+    if (sort_moves_rows<T, D>()) b += 2 * sizeof(T) * D * nn;   // rows between sort passes
     const size_t nj = static_cast<size_t>(njobs);
     const size_t mm_blocks = nn / static_cast<size_t>(minmax_rows_per_block()) + nj + 1;
     b += sizeof(T) * 2 * D * (mm_blocks + nj) + 4 * (mm_blocks + nj + 2) + 64;  // min/max
     b += sizeof(JobParams<T>) * nj + 4 * nn;   // job table, job of every cell
     if (one_thread) b += 8 * nn + 4 * nn + nn + 4;  // ambiguous rows, f, involved, flag
     b += 8 * (16 + kNoiseStripes);            // statistics, counters, noise counters
-    b += 52 * 256 + 64 * 1024;                 // alignment
+    b += 54 * 256 + 64 * 1024;                 // alignment
     return b;
 }
 
@@ -396,9 +397,21 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     tm.mark(M_GRID);
 
     // stable sort (spatial_index.h:167-216): slot -> input row; jobs stay
-    // contiguous and in order because their key ranges are
-    const bool in_b =
-        radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, key_bits + sub_bits, hist, tmp, s);
+    // contiguous and in order because their key ranges are.  Rows of up to
+    // 16 bytes travel with their pairs into soa (a gather of random rows
+    // afterwards reads a whole cache line per row).
+    const int sort_bits = key_bits + sub_bits;
+    const bool rows_sorted = sort_moves_rows<T, D>() && n > 1 && sort_bits > 0;
+    bool in_b = false;
+    if (rows_sorted) {
+        const int passes = radix_pass_count(sort_bits);
+        T* rows_a = passes > 1 ? A.take_n<T>(nn * D) : nullptr;
+        T* rows_b = passes > 2 ? A.take_n<T>(nn * D) : nullptr;
+        in_b = radix_sort_pairs_rows<K, T, D>(keys_a, vals_a, keys_b, vals_b, n, sort_bits, hist,
+                                              tmp, pts, rows_a, rows_b, soa, ld, s);
+    } else {
+        in_b = radix_sort_pairs<K>(keys_a, vals_a, keys_b, vals_b, n, sort_bits, hist, tmp, s);
+    }
     K* keys = in_b ? keys_b : keys_a;
     uint32_t* inrow_of_slot = in_b ? vals_b : vals_a;
     K* keys_free = in_b ? keys_a : keys_b;
@@ -436,9 +449,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         launch_box_table<K>(keys, n, flags, flags + n, vals_free,
                             reinterpret_cast<uint32_t*>(keys_free), tmp, bstart, bcell, box_of_slot,
                             &counters[3], nullptr, 2 * D, static_cast<int>(sizeof(T)), s);
-        launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, bpart,
-                                  s);
-    } else {
+        launch_gather_boxes<T, D>(rows_sorted ? nullptr : pts, n, inrow_of_slot, soa, ld,
+                                  box_of_slot, bstart, bbox, bpart, s);
+    } else if (!rows_sorted) {
         launch_gather_soa<T, D>(pts, n, inrow_of_slot, soa, ld, s);
     }
     // one read-back for the occupied cells and the boxes

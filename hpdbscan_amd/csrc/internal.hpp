@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -129,6 +130,22 @@ void radix_sort_pairs_dev(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals
 template <typename K>
 bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
                       int bits, uint32_t* hist, uint32_t* tmp, hipStream_t s);
+// radix_sort_pairs (n > 1, bits > 0, pairs of row index i -> row i) that
+// also moves the rows: row i of pts (D values) lands where its pair does,
+// in soa (D planes of stride ld); rows_a and rows_b (n * D values each; b
+// only with three passes or more) hold the rows between passes.  Reading
+// each tile's rows where it lies replaces a gather of random rows after
+// the sort.  Instantiated for rows of 4 or 8 bytes (float, D <= 2; double,
+// D = 1): wider rows would take LDS that the sort's occupancy needs.
+template <typename K, typename T, int D>
+bool radix_sort_pairs_rows(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                           int bits, uint32_t* hist, uint32_t* tmp, const T* pts, T* rows_a,
+                           T* rows_b, T* soa, int64_t ld, hipStream_t s);
+int radix_pass_count(int bits);  // passes of radix_sort_pairs over `bits` key bits
+template <typename T, int D>
+constexpr bool sort_moves_rows() {
+    return (std::is_same<T, float>::value && D <= 2) || (std::is_same<T, double>::value && D == 1);
+}
 
 // Grid geometry of the eps-grid, computed on the host from the min/max of
 // the data exactly as the reference does (spatial_index.h:124-142).

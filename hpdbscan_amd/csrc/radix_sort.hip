@@ -13,6 +13,7 @@
 // inside a tile is (wave, iteration, lane), i.e. input order.  9-bit digits
 // (512 buckets, two per thread) are used where they save a pass.
 #include "common.hpp"
+#include "hpdbscan_capi.h"
 #include "internal.hpp"
 
 namespace hpd {
@@ -53,7 +54,39 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ k
         hist[static_cast<int64_t>(dg) * ntiles + blockIdx.x] = cnt[dg];
 }
 
-template <typename K, int kBits>
+// Rows that a pass carries along with the pairs: row i of `in` (one word W
+// of D values of T) goes where pair i goes, into `out` (words), or on the
+// last pass into D planes of stride ld (the coordinates in sorted order).
+// The row is read with its key, staged in LDS with it and written with it.
+struct NoRows {
+    static constexpr bool kOn = false;
+    using W = uint32_t;
+    __device__ W load(int64_t) const { return 0; }
+    __device__ void store(int64_t, W) const {}
+};
+template <typename T, int D>
+struct MoveRows {
+    static constexpr bool kOn = true;
+    using W = std::conditional_t<sizeof(T) * D == 8, uint64_t, uint32_t>;
+    static_assert(sizeof(T) * D == sizeof(W), "a row is one 4- or 8-byte word");
+    const W* in;
+    W* out;      // ld == 0
+    T* planes;   // ld > 0
+    int64_t ld;
+    __device__ __forceinline__ W load(int64_t i) const { return in[i]; }
+    __device__ __forceinline__ void store(int64_t to, W v) const {
+        if (ld > 0) {
+            T x[D];
+            __builtin_memcpy(x, &v, sizeof(W));
+#pragma unroll
+            for (int j = 0; j < D; ++j) planes[static_cast<int64_t>(j) * ld + to] = x[j];
+        } else {
+            out[to] = v;
+        }
+    }
+};
+
+template <typename K, int kBits, typename Rows = NoRows>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             K* __restrict__ kout,
@@ -61,7 +94,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
                                                             int shift,
                                                             const uint32_t* __restrict__ offs,
                                                             int64_t ntiles,
-                                                            const int* __restrict__ bits_dev) {
+                                                            const int* __restrict__ bits_dev,
+                                                            Rows rows = Rows{}) {
     if (bits_dev && shift >= *bits_dev) return;  // a pass beyond the key's bits
     constexpr int B = 1 << kBits;
     constexpr uint32_t kMask = B - 1;
@@ -72,6 +106,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     __shared__ uint32_t wsum[kThreads / 64];
     __shared__ K skey[kTile];
     __shared__ uint32_t sval[kTile];
+    __shared__ typename Rows::W srow[Rows::kOn ? kTile : 1];
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
     for (int dg = threadIdx.x; dg < B; dg += kThreads) {
@@ -87,6 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     uint32_t val[kIters];
     uint32_t rank[kIters];
     uint32_t dgt[kIters];
+    typename Rows::W row[Rows::kOn ? kIters : 1];
 #pragma unroll
     for (int it = 0; it < kIters; ++it) {
         const int64_t idx = base + it * 64 + l;
@@ -107,6 +143,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
         dgt[it] = valid ? d : 0xffffffffu;
         key[it] = k;
         val[it] = valid ? vin[idx] : 0u;
+        if constexpr (Rows::kOn) row[it] = valid ? rows.load(idx) : 0;
     }
     __syncthreads();
     // exclusive prefix of each digit over the four waves, and of the digits'
@@ -153,6 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
             const uint32_t at = tbase[dgt[it]] + cnt[w][dgt[it]] + rank[it];
             skey[at] = key[it];
             sval[at] = val[it];
+            if constexpr (Rows::kOn) srow[at] = row[it];
         }
     }
     __syncthreads();
@@ -169,6 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
             const uint32_t dst = gbase[d] + (static_cast<uint32_t>(at) - tbase[d]);
             kout[dst] = k;
             vout[dst] = sval[at];
+            if constexpr (Rows::kOn) rows.store(dst, srow[at]);
         }
     }
 }
@@ -221,7 +260,53 @@ bool sort_passes(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64
     return in_b;
 }
 
+// sort_passes carrying rows: pass 0 reads them from `pts`, passes in
+// between alternate between rows_a and rows_b, the last writes the planes
+template <typename K, int kBits, typename T, int D>
+bool sort_passes_rows(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                      int bits, uint32_t* hist, uint32_t* tmp, const T* pts, T* rows_a,
+                      T* rows_b, T* soa, int64_t ld, hipStream_t s) {
+    const int64_t nt = tiles_of(n);
+    const int64_t nh = static_cast<int64_t>(1 << kBits) * nt;
+    K* kin = keys_a;
+    uint32_t* vin = vals_a;
+    K* kout = keys_b;
+    uint32_t* vout = vals_b;
+    bool in_b = false;
+    const T* rin = pts;
+    T* rnext = rows_a;
+    for (int shift = 0; shift < bits; shift += kBits) {
+        const bool last = shift + kBits >= bits;
+        using W = typename MoveRows<T, D>::W;
+        MoveRows<T, D> mv{reinterpret_cast<const W*>(rin), reinterpret_cast<W*>(rnext), soa,
+                          last ? ld : 0};
+        hipLaunchKernelGGL((k_radix_hist<K, kBits>), dim3(static_cast<unsigned>(nt)),
+                           dim3(kThreads), 0, s, kin, n, shift, hist, nt, nullptr);
+        exclusive_scan_u32(hist, hist, nh, tmp, nullptr, s);
+        hipLaunchKernelGGL((k_radix_scatter<K, kBits, MoveRows<T, D>>),
+                           dim3(static_cast<unsigned>(nt)), dim3(kThreads), 0, s, kin, vin, kout,
+                           vout, n, shift, hist, nt, nullptr, mv);
+        HPD_HIP(hipGetLastError());
+        K* tk = kin;
+        kin = kout;
+        kout = tk;
+        uint32_t* tv = vin;
+        vin = vout;
+        vout = tv;
+        in_b = !in_b;
+        rin = rnext;
+        rnext = rnext == rows_a ? rows_b : rows_a;
+    }
+    return in_b;
+}
+
 }  // namespace
+
+int radix_pass_count(int bits) {
+    if (bits <= 0) return 0;
+    const int digit = (bits + 8) / 9 < (bits + 7) / 8 ? 9 : 8;
+    return (bits + digit - 1) / digit;
+}
 
 size_t radix_hist_words(int64_t n) {
     return static_cast<size_t>(kMaxBuckets) * static_cast<size_t>(tiles_of(n > 0 ? n : 1));
@@ -236,6 +321,28 @@ bool radix_sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
         return sort_passes<K, 9>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
     return sort_passes<K, 8>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, s);
 }
+
+template <typename K, typename T, int D>
+bool radix_sort_pairs_rows(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
+                           int bits, uint32_t* hist, uint32_t* tmp, const T* pts, T* rows_a,
+                           T* rows_b, T* soa, int64_t ld, hipStream_t s) {
+    if (n <= 1 || bits <= 0)
+        throw Error(HPD_EINVAL, "radix_sort_pairs_rows needs n > 1 and bits > 0");
+    if ((bits + 8) / 9 < (bits + 7) / 8)
+        return sort_passes_rows<K, 9, T, D>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp,
+                                            pts, rows_a, rows_b, soa, ld, s);
+    return sort_passes_rows<K, 8, T, D>(keys_a, vals_a, keys_b, vals_b, n, bits, hist, tmp, pts,
+                                        rows_a, rows_b, soa, ld, s);
+}
+
+#define HPD_INST_SORT_ROWS(K, T, D)                                                              \
+    template bool radix_sort_pairs_rows<K, T, D>(K*, uint32_t*, K*, uint32_t*, int64_t, int,     \
+                                                 uint32_t*, uint32_t*, const T*, T*, T*, T*,     \
+                                                 int64_t, hipStream_t);
+#define HPD_INST_SORT_ROWS_K(T, D) HPD_INST_SORT_ROWS(uint32_t, T, D) HPD_INST_SORT_ROWS(uint64_t, T, D)
+HPD_INST_SORT_ROWS_K(float, 1)
+HPD_INST_SORT_ROWS_K(float, 2)
+HPD_INST_SORT_ROWS_K(double, 1)
 
 template <typename K>
 void radix_sort_pairs_dev(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,

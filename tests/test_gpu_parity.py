@@ -205,6 +205,36 @@ def test_open_pair_limit_takes_the_full_join(ctx, d):
     assert_parity(x, eps, mp, got, oracle.cluster(x, eps, mp))
 
 
+@pytest.mark.parametrize("d,dtype,mp", [(2, np.float32, 2000), (3, np.float64, 400)])
+def test_big_boxes_gathered_in_parts(ctx, d, dtype, mp):
+    """A sub-box of 70k rows (over the 65536 from which a box's rows are
+    gathered in parts across all such boxes) in the grid's corner, inside
+    sparse rows whose neighbour counts sit around minPts: core flags,
+    clusters and borders next to the big box equal the oracle's, and the
+    static pipeline's (whose gather takes every box whole)."""
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(70 + d)
+    big = rng.uniform(0.0, 0.45, (70_000, d))
+    sparse = rng.uniform(0.0, 6.0, (20_000, d))
+    x = np.concatenate([big, sparse])[rng.permutation(90_000)].astype(dtype)
+    lo = x.min(axis=0)
+    assert np.count_nonzero(np.all(x - lo < 0.5, axis=1)) >= 70_000  # one sub-box
+    say(f"big boxes d={d} {np.dtype(dtype).name} n=90000 eps=1 minPts={mp}")
+    got = ctx.cluster(x, 1.0, mp)
+    r = oracle.check_parallel(x, 1.0, mp, got, threads=16)
+    assert r["mismatches"] == 0 and r["bad_ambiguous"] == 0, r
+    cores = int(np.count_nonzero(got < 0))
+    assert r["cores"] == cores and 70_000 < cores < 90_000
+    assert (got > 0).any() and (got == 0).any()
+    g = _lib.Context(0)
+    try:
+        g.set_graph(True)
+        np.testing.assert_array_equal(g.cluster(x, 1.0, mp), got)
+        assert g.graph_stats()["fallbacks"] == 0
+    finally:
+        g.close()
+
+
 def dense_blobs(n, d, seed):
     """Four tight blobs on separated centres with a little noise around them:
     few occupied cells, each holding many rows."""
