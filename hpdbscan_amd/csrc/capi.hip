@@ -728,11 +728,18 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         HPD_HIP(hipMemsetAsync(involved, 0, nn, s));
         launch_one_thread_mark<T, D>(a, core, rootslot, amb, namb, involved, f, s);
         // sweep k leaves every core whose best decreasing path has at most
-        // k steps final, so sweep n + 1 at the latest changes nothing
-        for (int64_t sweep = 0;; ++sweep) {
-            if (sweep > n + 1) throw Error(HPD_EDEVICE, "border relaxation did not converge");
-            HPD_HIP(hipMemsetAsync(changed, 0, sizeof(unsigned int), s));
-            launch_one_thread_relax<T, D>(a, core, rootslot, involved, f, changed, s);
+        // k steps final, so sweep n + 1 at the latest changes nothing.  A
+        // sweep that changes nothing is a fixed point; the host reads the
+        // flag of the last sweep of a batch, and the batches double (up to
+        // 64 sweeps), so a long chain costs few read-backs
+        int64_t sweeps = 0;
+        for (int batch = 1;; batch = std::min(2 * batch, 64)) {
+            if (sweeps > n + 1) throw Error(HPD_EDEVICE, "border relaxation did not converge");
+            for (int i = 0; i < batch; ++i) {
+                if (i == batch - 1) HPD_HIP(hipMemsetAsync(changed, 0, sizeof(unsigned int), s));
+                launch_one_thread_relax<T, D>(a, core, rootslot, involved, f, changed, s);
+            }
+            sweeps += batch;
             HPD_HIP(hipMemcpyAsync(hc + 32, changed, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
             HPD_HIP(hipStreamSynchronize(s));
             if (hc[32] == 0) break;
