@@ -217,6 +217,7 @@ def batched_leg(a) -> dict:
     if a.graph:
         gctx = _lib.Context(0)
         gctx.set_work_split(*ctx.work_split())
+        gctx.set_box_split(ctx.box_split())
         gctx.set_graph(True)
         gl = _lib.DeviceBuffer(gctx, 8 * len(rows))
         t = time.perf_counter()
@@ -269,8 +270,8 @@ def batched_leg(a) -> dict:
         "batched_latency_ms": round(t_batch * 1e3, 2),
         "speedup_jobs_per_s": round(t_plain / t_batch, 2),
         "labels_equal": equal,
-        "work_split": tuned["best"] if tuned else dict(zip(("sparse_max", "sparse_core"),
-                                                           (8, False))),
+        "work_split": tuned["best"] if tuned else {"box_split": 0, "sparse_max": 8,
+                                                   "sparse_core": False},
         "note": "device-resident rows; a batched job's latency is the whole batch's",
         **graph,
     }

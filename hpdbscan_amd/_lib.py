@@ -112,6 +112,8 @@ SIGNATURES = {
     "hpd_context_set_border_rule": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hpd_context_set_work_split": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int]),
     "hpd_context_set_open_pair_limit": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "hpd_context_set_box_split": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "hpd_context_get_box_split": (ctypes.c_int, [_vp, _vp]),
     "hpd_context_get_work_split": (ctypes.c_int, [_vp, _vp, _vp]),
     "hpd_context_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hpd_context_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -227,6 +229,16 @@ class Context:
         """Most open box pairs the union lists hold (0: 2n); beyond it the
         union walks every box's neighbourhood.  Results do not change."""
         check(load().hpd_context_set_open_pair_limit(self._h, int(limit)))
+
+    def set_box_split(self, parts: int) -> None:
+        """Sub-boxes per cell and dimension (0: 2 up to 3-D, 3 above; 1-8
+        set it).  Results do not change."""
+        check(load().hpd_context_set_box_split(self._h, int(parts)))
+
+    def box_split(self) -> int:
+        k = ctypes.c_int32()
+        check(load().hpd_context_get_box_split(self._h, ctypes.byref(k)))
+        return int(k.value)
 
     def work_split(self) -> tuple[int, bool]:
         sm, sc = ctypes.c_int32(), ctypes.c_int()

@@ -1015,8 +1015,9 @@ __global__ __launch_bounds__(kThreads) void k_iota_i32(int32_t* __restrict__ a, 
 
 int box_split(int d) { return d <= 3 ? 2 : 3; }
 
-int box_sub_bits(int d) {
-    const int k = box_split(d);
+int box_sub_bits(int d) { return box_sub_bits(d, box_split(d)); }
+
+int box_sub_bits(int d, int k) {
     uint64_t v = 1;
     for (int j = 0; j < d; ++j) v *= static_cast<uint64_t>(k);
     const uint64_t mx = v - 1;

@@ -12,6 +12,7 @@ namespace hpd {
 // boxes per cell and dimension (k) and bits of the sub-box digit
 int box_split(int d);
 int box_sub_bits(int d);
+int box_sub_bits(int d, int k);  // bits of the sub-box digit with k parts per dimension
 
 template <typename T>
 struct BoxArgs {

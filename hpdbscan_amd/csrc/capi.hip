@@ -223,6 +223,9 @@ struct Context {
     // the others one wave per 64-row chunk (hpd_context_set_work_split;
     // defaults picked by hpdbscan_amd.autotune on the 10M x 3-D workload)
     int32_t sparse_max = 8;
+    // sub-boxes per cell and dimension (hpd_context_set_box_split; 0: 2 up
+    // to 3-D, 3 above, which keeps every sub-box's diagonal below eps)
+    int32_t box_split = 0;
     uint64_t open_pair_limit = 0;  // 0: 2n
     bool sparse_core = false;
     bool timing = false;
@@ -234,7 +237,7 @@ struct Context {
     bool graph = false;
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
-        int dtype = -1, d = 0, border_rule = -1, sparse_max = -1;
+        int dtype = -1, d = 0, border_rule = -1, sparse_max = -1, box_split = -1;
         bool sparse_core = false;
         int64_t n = 0;
         const void* data = nullptr;
@@ -318,6 +321,9 @@ static void check_neighbor_rows(uint64_t total) {
                         "occupied cells); at most 2^31-1 are supported");
 }
 
+// sub-boxes per cell and dimension of a run of dimension d
+static int split_of(const Context& c, int d) { return c.box_split > 0 ? c.box_split : box_split(d); }
+
 static inline int bits_for(uint64_t max_value) {
     return max_value == 0 ? 0 : 64 - __builtin_clzll(max_value);
 }
@@ -392,8 +398,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     // cell ids (spatial_index.h:144-165), offset by the job's key base; in
     // box mode followed by the row's sub-box digit, so that one stable sort
     // orders the rows by cell, then sub-box, then input row
-    const int sub_bits = boxes ? box_sub_bits(D) : 0;
-    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, box_split(D), sub_bits, keys_a, vals_a, s);
+    const int sub_k = split_of(ctx, D);
+    const int sub_bits = boxes ? box_sub_bits(D, sub_k) : 0;
+    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, sub_k, sub_bits, keys_a, vals_a, s);
     tm.mark(M_GRID);
 
     // stable sort (spatial_index.h:167-216): slot -> input row; jobs stay
@@ -696,7 +703,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                            key_total <= (nn * sizeof(K) / sizeof(int32_t)) >> sub_bits;
     unsigned int* full_ok = full_rows ? &counters[13] : nullptr;
     if (full_rows)
-        launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, host_count(nbox),
+        launch_label_full_rows<T, D, K>(pts, a, sub_k, sub_bits, keys, host_count(nbox),
                                         bcluster, reinterpret_cast<int32_t*>(keys_free),
                                         static_cast<int64_t>(nn * sizeof(K) / 4), labels,
                                         full_ok, s);
@@ -849,7 +856,8 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     unsigned long long* total_cells = dstats + 6;
     launch_minmax<T, D>(pts, jobs, njobs, bfirst_dev, nblocks, part, mm,
                         reinterpret_cast<int*>(&sc[SS_NONFINITE]), s);
-    const int sub_bits = box_sub_bits(D);
+    const int sub_k = split_of(ctx, D);
+    const int sub_bits = box_sub_bits(D, sub_k);
     launch_geometry<T, D>(jobs, njobs, mm, sub_bits, jcells, reinterpret_cast<int*>(&sc[SS_GRID]),
                           reinterpret_cast<int*>(&sc[SS_BITS]), total_cells, s);
 
@@ -892,7 +900,7 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     uint32_t* ccnt = A.take_n<uint32_t>(nn + 1);
 
     // cell ids and the sort (passes past the key's bits do nothing)
-    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, box_split(D), sub_bits, keys_a, vals_a, s);
+    launch_cell_ids<T, D, K>(pts, n, jobs, njobs, sub_k, sub_bits, keys_a, vals_a, s);
     radix_sort_pairs_dev<K>(keys_a, vals_a, keys_b, vals_b, n, 64,
                             reinterpret_cast<const int*>(&sc[SS_BITS]), hist, tmp, s);
     K* keys = keys_a;
@@ -1030,7 +1038,7 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
 
     // the rows of full boxes labelled in input order when the keys fit the
     // table (in the sort's spare key buffer), the other cores in slot order
-    launch_label_full_rows<T, D, K>(pts, a, box_split(D), sub_bits, keys, NB, bcluster,
+    launch_label_full_rows<T, D, K>(pts, a, sub_k, sub_bits, keys, NB, bcluster,
                                     reinterpret_cast<int32_t*>(keys_b),
                                     static_cast<int64_t>(nn * sizeof(K) / 4), labels,
                                     &sc[SS_ROWTAB], s);
@@ -1100,6 +1108,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
                       ge.labels == labels && ge.arena_main == A.base() &&
                       ge.arena_aux == ctx.aux.base() && ge.border_rule == ctx.border_rule &&
                       ge.sparse_max == ctx.sparse_max && ge.sparse_core == ctx.sparse_core &&
+                      ge.box_split == ctx.box_split &&
                       ge.offsets.size() == static_cast<size_t>(njobs) + 1 &&
                       std::equal(ge.offsets.begin(), ge.offsets.end(), offsets);
     if (!same) {
@@ -1130,6 +1139,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
         ge.arena_aux = ctx.aux.base();
         ge.border_rule = ctx.border_rule;
         ge.sparse_max = ctx.sparse_max;
+        ge.box_split = ctx.box_split;
         ge.sparse_core = ctx.sparse_core;
         ge.offsets.assign(offsets, offsets + njobs + 1);
         ++ctx.graph_captures;
@@ -1155,7 +1165,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
     ctx.stats.ambiguous_borders = hs[3];
     ctx.stats.open_box_pairs = hs[5];
     ctx.stats.total_cells = hs[6];
-    ctx.stats.key_bits = static_cast<int32_t>(hc[SS_BITS]) - box_sub_bits(D);
+    ctx.stats.key_bits = static_cast<int32_t>(hc[SS_BITS]) - box_sub_bits(D, split_of(ctx, D));
     ctx.stats.occupied_cells = hc[SS_M];
     ctx.stats.neighbor_rows = hc[SS_RANGES];
     ctx.stats.boxes = hc[SS_NBOX];
@@ -1258,8 +1268,9 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
     // floating-point data runs on sub-boxes (the sort key then carries the
     // sub-box digit below the cell id); integer data, or a grid so large
     // that the digit does not fit beside the cell id, runs on cells
-    const bool boxes = std::is_floating_point<T>::value && key_bits + box_sub_bits(D) <= 64;
-    const int sort_bits = key_bits + (boxes ? box_sub_bits(D) : 0);
+    const int split_bits = box_sub_bits(D, split_of(ctx, D));
+    const bool boxes = std::is_floating_point<T>::value && key_bits + split_bits <= 64;
+    const int sort_bits = key_bits + (boxes ? split_bits : 0);
     if (sort_bits <= 32)
         run_keyed<T, D, uint32_t>(ctx, pts, n, jobs_dev, njobs, key_bits, key_total, boxes, labels,
                                   s, tm);
@@ -1572,6 +1583,21 @@ int hpd_context_set_work_split(hpd_context* ctx, int32_t sparse_max, int sparse_
             throw Error(HPD_EINVAL, "sparse_max must be in [0, 64]");
         ctx->impl.sparse_max = sparse_max;
         ctx->impl.sparse_core = sparse_core != 0;
+    });
+}
+
+int hpd_context_set_box_split(hpd_context* ctx, int32_t parts) {
+    return guarded([&] {
+        if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
+        if (parts < 0 || parts > 8) throw Error(HPD_EINVAL, "box split must be in [0, 8]");
+        ctx->impl.box_split = parts;
+    });
+}
+
+int hpd_context_get_box_split(const hpd_context* ctx, int32_t* parts) {
+    return guarded([&] {
+        if (!ctx || !parts) throw Error(HPD_EINVAL, "NULL argument");
+        *parts = ctx->impl.box_split;
     });
 }
 

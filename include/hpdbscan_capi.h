@@ -149,6 +149,16 @@ int hpd_context_get_work_split(const hpd_context* ctx, int32_t* sparse_max, int*
  * Results do not depend on it.  No counterpart in the reference. */
 int hpd_context_set_open_pair_limit(hpd_context* ctx, uint64_t limit);
 
+/* Sub-boxes per cell and dimension (the tile of the pair kernels): each
+ * eps-cell is split into parts^d boxes, each box's bounding box deciding
+ * whole groups of distance tests.  0 (the default) takes 2 up to 3-D and 3
+ * above, which keeps every box's diagonal below eps; 1 to 8 set it.  Boxes
+ * wider than eps fall back to the row-level union.  Results do not depend
+ * on it; hpdbscan_amd.autotune picks it for a workload.  No counterpart in
+ * the reference (its cells are the only tiles, spatial_index.h:144-165). */
+int hpd_context_set_box_split(hpd_context* ctx, int32_t parts);
+int hpd_context_get_box_split(const hpd_context* ctx, int32_t* parts);
+
 /*
  * Cluster n rows of d values of type `dtype` (row-major, host memory).
  * labels_out (host, n entries) receives the reference's labels in input

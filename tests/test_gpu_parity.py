@@ -235,6 +235,46 @@ def test_big_boxes_gathered_in_parts(ctx, d, dtype, mp):
         g.close()
 
 
+@pytest.mark.parametrize("d,dtype", [(1, np.float32), (2, np.float32), (3, np.float32),
+                                     (4, np.float64), (6, np.float32)])
+def test_box_split_never_changes_labels(d, dtype):
+    """Every box split (1 to 5 sub-boxes per cell and dimension; 1: boxes
+    are cells, wider than eps from 2-D on, which takes the row-level union)
+    gives the labels of the default split, which equal the oracle's; with
+    the one-thread border rule every label is the oracle's; the static
+    pipeline takes the split too."""
+    from hpdbscan_amd import _lib
+    box, eps, mp = {1: (40.0, 0.02, 5), 2: (6.0, 0.1, 8), 3: (8.0, 0.3, 8), 4: (8.0, 0.5, 8),
+                    6: (6.0, 0.9, 6)}[d]
+    x = blobs(20000, d, 15, 0.15, box, 80 + d, dtype)
+    say(f"box splits d={d} {np.dtype(dtype).name} n=20000 eps={eps} minPts={mp}")
+    want = oracle.cluster(x, eps, mp)
+    c = _lib.Context(0)
+    c1 = _lib.Context(0)
+    g = _lib.Context(0)
+    try:
+        c1.set_border_rule(_lib.BORDER_ONE_THREAD)
+        g.set_graph(True)
+        base = c.cluster(x, eps, mp)
+        assert_parity(x, eps, mp, base, want)
+        for k in (1, 2, 3, 4, 5):
+            if k ** d > 1 << 24:
+                continue
+            c.set_box_split(k)
+            assert c.box_split() == k
+            np.testing.assert_array_equal(c.cluster(x, eps, mp), base, err_msg=f"split {k}")
+            c1.set_box_split(k)
+            np.testing.assert_array_equal(c1.cluster(x, eps, mp), want, err_msg=f"split {k}")
+        g.set_box_split(3)
+        np.testing.assert_array_equal(g.cluster(x, eps, mp), base)
+        with pytest.raises(ValueError):
+            c.set_box_split(9)
+    finally:
+        c.close()
+        c1.close()
+        g.close()
+
+
 def dense_blobs(n, d, seed):
     """Four tight blobs on separated centres with a little noise around them:
     few occupied cells, each holding many rows."""
