@@ -29,7 +29,8 @@ The JSON line also carries
                 hipGraph and replayed (hpd_context_set_graph)
   serving       a Poisson stream (200 requests/s, 10k-5M rows, d in 2..4)
                 through the continuous-batching server: offered vs achieved
-                load, p50/p99 latency
+                load, p50/p99 latency; the same stream device-resident
+                (hpd_server_submit_device) and at 2x and 4x its rate
 Only one GPU is used; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
@@ -306,31 +307,25 @@ def _small_batch(ctx, gctx, rows, off, eps, mps, reps, jobs=256) -> dict:
     return out
 
 
-def serving_leg(a) -> dict:
-    """Poisson arrivals into the continuous-batching server."""
+def _serve_stream(a, stream, scale=1.0, device=None) -> dict:
+    """Replay `stream` (arrival times multiplied by `scale`) into a fresh
+    server; `device`: per request (rows, labels) DeviceBuffers, submitted
+    with hpd_server_submit_device.  Returns the latency / throughput figures."""
     import threading
-    t0 = time.time()
-    stream = datagen.serving_stream(n_requests=a.serve_requests, rate=a.serve_rate)
-    offered_rows = sum(len(r[1]) for r in stream)
-    span = stream[-1][0] if stream else 0.0
-    print(f"[bench] serving: {len(stream)} requests over {span:.2f}s, {offered_rows} rows "
-          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    srv = _lib.Server(lanes=a.serve_lanes)
-    # warm-up request per dimension (contexts, allocations)
-    for d in (2, 3, 4):
-        w = next(r for r in stream if r[1].shape[1] == d)
-        srv.wait(srv.submit(w[1], w[2], w[3]))
-    srv.close()
     srv = _lib.Server(lanes=a.serve_lanes)
     tickets = [None] * len(stream)
     start = time.perf_counter()
 
     def producer():
         for i, (arr, x, e, m) in enumerate(stream):
-            delay = start + arr - time.perf_counter()
+            delay = start + arr * scale - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            tickets[i] = srv.submit(x, e, m)
+            if device is None:
+                tickets[i] = srv.submit(x, e, m)
+            else:
+                xd, ld = device[i]
+                tickets[i] = srv.submit_device(xd.ptr, len(x), x.shape[1], x.dtype, e, m, ld.ptr)
 
     th = threading.Thread(target=producer)
     th.start()
@@ -343,17 +338,71 @@ def serving_leg(a) -> dict:
     st = srv.stats()
     srv.close()
     lat = np.array([i["latency_ms"] for i in infos])
+    span = stream[-1][0] * scale if stream else 0.0
+    rows = sum(len(r[1]) for r in stream)
     return {
-        "workload": "serving_stream", "requests": len(stream), "lanes": a.serve_lanes,
         "offered_jobs_per_s": round(len(stream) / span, 1) if span > 0 else None,
-        "offered_mpoints_per_s": round(offered_rows / span / 1e6, 1) if span > 0 else None,
+        "offered_mpoints_per_s": round(rows / span / 1e6, 1) if span > 0 else None,
         "achieved_jobs_per_s": round(len(stream) / wall, 1),
-        "achieved_mpoints_per_s": round(offered_rows / wall / 1e6, 1),
+        "achieved_mpoints_per_s": round(rows / wall / 1e6, 1),
         "p50_ms": round(float(np.percentile(lat, 50)), 2),
         "p99_ms": round(float(np.percentile(lat, 99)), 2),
         "max_ms": round(float(lat.max()), 2),
         "batches": int(st["batches"]), "mean_batch_jobs": round(st["mean_batch_jobs"], 2),
     }
+
+
+def serving_leg(a) -> dict:
+    """Poisson arrivals into the continuous-batching server: host requests
+    at the workload's rate, the same stream device-resident (rows and labels
+    in HBM, hpd_server_submit_device), and host requests at 2x and 4x the
+    rate (the same arrivals compressed)."""
+    t0 = time.time()
+    stream = datagen.serving_stream(n_requests=a.serve_requests, rate=a.serve_rate)
+    offered_rows = sum(len(r[1]) for r in stream)
+    span = stream[-1][0] if stream else 0.0
+    print(f"[bench] serving: {len(stream)} requests over {span:.2f}s, {offered_rows} rows "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    srv = _lib.Server(lanes=a.serve_lanes)
+    # warm-up request per dimension (contexts, allocations)
+    for d in (2, 3, 4):
+        w = next(r for r in stream if r[1].shape[1] == d)
+        srv.wait(srv.submit(w[1], w[2], w[3]))
+    srv.close()
+    out = {"workload": "serving_stream", "requests": len(stream), "lanes": a.serve_lanes,
+           **_serve_stream(a, stream)}
+    # the same stream with every request's rows and labels in device memory
+    ctx = _lib.Context(0)
+    device = [(_lib.DeviceBuffer.from_array(ctx, x), _lib.DeviceBuffer(ctx, 8 * len(x)))
+              for _, x, _, _ in stream]
+    sweep = []
+    try:
+        dev = _serve_stream(a, stream, device=device)
+        # a sample of the device requests' labels against plain calls
+        for i in range(0, len(stream), max(1, len(stream) // 8)):
+            _, x, e, m = stream[i]
+            if not np.array_equal(device[i][1].to_array(np.int64, len(x)), ctx.cluster(x, e, m)):
+                dev["labels_equal_sample"] = False
+                break
+        else:
+            dev["labels_equal_sample"] = True
+        print(f"[bench] serving device-resident: p50 {dev['p50_ms']} ms, p99 {dev['p99_ms']} ms",
+              file=sys.stderr, flush=True)
+        for mult in a.serve_sweep:
+            for kind, bufs in (("host", None), ("device", device)):
+                r = _serve_stream(a, stream, scale=1.0 / mult, device=bufs)
+                sweep.append({"rate_multiple": mult, "requests_in": kind, **r})
+                print(f"[bench] serving {kind} at {mult}x: achieved {r['achieved_jobs_per_s']} of "
+                      f"{r['offered_jobs_per_s']} jobs/s, p50 {r['p50_ms']} ms, "
+                      f"p99 {r['p99_ms']} ms", file=sys.stderr, flush=True)
+    finally:
+        for xd, ld in device:
+            xd.close()
+            ld.close()
+        ctx.close()
+    out["device_resident"] = dev
+    out["rate_sweep"] = sweep
+    return out
 
 
 def device_steps(ctx, x, eps, mp, steps, warmup, timing=True):
@@ -550,6 +599,8 @@ def main() -> None:
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[2.0, 4.0],
+                    help="serving: also replay the stream at these multiples of its rate")
     a = ap.parse_args()
 
     if a.gpus != 1:

@@ -144,6 +144,9 @@ SIGNATURES = {
     "hpd_server_submit": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int32,
                                          ctypes.c_float, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "hpd_server_submit_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64,
+                                                ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
+                                                _vp, ctypes.POINTER(ctypes.c_uint64)]),
     "hpd_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, ctypes.c_int64, ctypes.c_double,
                                        ctypes.POINTER(RequestInfo)]),
     "hpd_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
@@ -457,13 +460,25 @@ class Server:
         self._n[t.value] = a.shape[0]
         return t.value
 
+    def submit_device(self, data_ptr: int, n: int, d: int, dtype, eps: float, min_points: int,
+                      labels_ptr: int) -> int:
+        """A request whose rows (n x d of dtype) and labels (n int64) are in
+        device memory; both must stay valid until wait returns."""
+        t = ctypes.c_uint64()
+        _check_server(load().hpd_server_submit_device(self._h, DTYPE_CODES[np.dtype(dtype)],
+                                                      data_ptr, int(n), int(d), ctypes.c_float(eps),
+                                                      int(min_points), labels_ptr, ctypes.byref(t)))
+        self._n[t.value] = None
+        return t.value
+
     def wait(self, ticket: int, timeout_ms: float = -1.0):
-        """(labels, info dict) of a ticket."""
+        """(labels, info dict) of a ticket; labels None for a device request
+        (they are in its labels buffer)."""
         n = self._n[ticket]
-        out = np.empty(n, dtype=np.int64)
+        out = None if n is None else np.empty(n, dtype=np.int64)
         info = RequestInfo()
-        _check_server(load().hpd_server_wait(self._h, ticket, out.ctypes.data, n, timeout_ms,
-                                             ctypes.byref(info)))
+        _check_server(load().hpd_server_wait(self._h, ticket, None if out is None else out.ctypes.data,
+                                             0 if n is None else n, timeout_ms, ctypes.byref(info)))
         del self._n[ticket]
         return out, info.as_dict()
 

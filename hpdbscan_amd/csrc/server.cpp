@@ -13,6 +13,15 @@
 // share of a batch needs (rows * bytes_per_row); a batch is dispatched only
 // while the charges of all running batches plus its own stay within the
 // budget.  A request that cannot fit even alone fails with HPD_ENOMEM.
+//
+// Data movement: a lane keeps grow-only device buffers for a batch's rows
+// and labels and pinned host buffers to stage them.  A host request's rows
+// go host -> pinned -> device and its labels back the same way; a device
+// request (hpd_server_submit_device) has its rows copied device to device
+// into the batch and its labels copied device to device into the caller's
+// buffer, so its data never leaves HBM.
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -50,8 +59,11 @@ struct Request {
     int64_t n = 0;
     float eps = 0.0f;
     uint64_t min_pts = 0;
-    std::vector<char> rows;       // copy of the caller's rows
-    std::vector<int64_t> labels;  // result
+    std::vector<char> rows;       // copy of the caller's rows (host requests)
+    std::vector<int64_t> labels;  // result (host requests)
+    const void* d_rows = nullptr; // device requests: the caller's rows ...
+    int64_t* d_labels = nullptr;  // ... and labels, in device memory
+    bool device = false;
     size_t charge = 0;            // device bytes reserved for it
     int status = HPD_OK;
     std::string error;
@@ -66,7 +78,42 @@ struct Lane {
     std::vector<std::shared_ptr<Request>> work;  // current batch
     bool busy = false;
     bool has_work = false;
+    // grow-only batch buffers: device rows / labels, pinned staging
+    void* d_rows = nullptr;
+    int64_t* d_labels = nullptr;
+    char* h_rows = nullptr;
+    int64_t* h_labels = nullptr;
+    size_t rows_cap = 0, labels_cap = 0, h_rows_cap = 0, h_labels_cap = 0;
 };
+
+// Grow a device (pinned: host) buffer to at least `bytes` (with a quarter
+// of headroom, so a stream of growing batches reallocates rarely).
+hipError_t grow(void** p, size_t* cap, size_t bytes, bool pinned) {
+    if (bytes <= *cap) return hipSuccess;
+    const size_t want = bytes + bytes / 4;
+    if (*p) {
+        if (pinned)
+            (void)hipHostFree(*p);
+        else
+            (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    const hipError_t e = pinned ? hipHostMalloc(p, want, hipHostMallocDefault) : hipMalloc(p, want);
+    if (e == hipSuccess) *cap = want;
+    return e;
+}
+
+void release(Lane& l) {
+    if (l.d_rows) (void)hipFree(l.d_rows);
+    if (l.d_labels) (void)hipFree(l.d_labels);
+    if (l.h_rows) (void)hipHostFree(l.h_rows);
+    if (l.h_labels) (void)hipHostFree(l.h_labels);
+    l.d_rows = nullptr;
+    l.d_labels = nullptr;
+    l.h_rows = nullptr;
+    l.h_labels = nullptr;
+}
 
 }  // namespace
 
@@ -105,6 +152,76 @@ size_t request_charge(const hpd_server& s, const Request& r) {
            static_cast<size_t>(r.n) * static_cast<size_t>(r.d) * dtype_bytes(r.dtype);
 }
 
+// One batched call for the lane's batch: rows into the lane's device
+// buffer (host requests through the pinned buffer, device requests device
+// to device), hpd_cluster_batch_device, labels out (host requests into the
+// pinned labels, device requests into the caller's buffers).
+int run_batch(hpd_server* srv, Lane* lane, const std::vector<std::shared_ptr<Request>>& work,
+              const std::vector<int64_t>& offsets, const std::vector<float>& eps,
+              const std::vector<uint64_t>& mps, int dtype, int32_t d, size_t row_bytes,
+              int64_t total, std::string* err) {
+    auto hip_fail = [&](hipError_t e, const char* what) {
+        *err = std::string(what) + ": " + hipGetErrorString(e);
+        return HPD_EDEVICE;
+    };
+    hipError_t e = hipSetDevice(srv->cfg.device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    void* sp = nullptr;
+    if (hpd_context_stream(lane->ctx, &sp) != HPD_OK) {
+        *err = hpd_last_error();
+        return HPD_EDEVICE;
+    }
+    hipStream_t stream = static_cast<hipStream_t>(sp);
+    const size_t nrows = static_cast<size_t>(total);
+    bool any_host = false;
+    for (const auto& r : work) any_host = any_host || !r->device;
+    if ((e = grow(&lane->d_rows, &lane->rows_cap, std::max<size_t>(1, nrows * row_bytes), false)) !=
+            hipSuccess ||
+        (e = grow(reinterpret_cast<void**>(&lane->d_labels), &lane->labels_cap,
+                  std::max<size_t>(1, nrows) * sizeof(int64_t), false)) != hipSuccess)
+        return hip_fail(e, "hipMalloc of the batch buffers");
+    if (any_host &&
+        ((e = grow(reinterpret_cast<void**>(&lane->h_rows), &lane->h_rows_cap,
+                   std::max<size_t>(1, nrows * row_bytes), true)) != hipSuccess ||
+         (e = grow(reinterpret_cast<void**>(&lane->h_labels), &lane->h_labels_cap,
+                   std::max<size_t>(1, nrows) * sizeof(int64_t), true)) != hipSuccess))
+        return hip_fail(e, "hipHostMalloc of the staging buffers");
+    char* drows = static_cast<char*>(lane->d_rows);
+    for (size_t j = 0; j < work.size(); ++j) {
+        const Request& r = *work[j];
+        const size_t off = static_cast<size_t>(offsets[j]) * row_bytes;
+        const size_t bytes = static_cast<size_t>(r.n) * row_bytes;
+        if (bytes == 0) continue;
+        if (r.device) {
+            e = hipMemcpyAsync(drows + off, r.d_rows, bytes, hipMemcpyDeviceToDevice, stream);
+        } else {
+            std::memcpy(lane->h_rows + off, r.rows.data(), bytes);
+            e = hipMemcpyAsync(drows + off, lane->h_rows + off, bytes, hipMemcpyHostToDevice,
+                               stream);
+        }
+        if (e != hipSuccess) return hip_fail(e, "copy of a request's rows");
+    }
+    const int rc = hpd_cluster_batch_device(lane->ctx, dtype, lane->d_rows, offsets.data(),
+                                            static_cast<int32_t>(work.size()), d, eps.data(),
+                                            mps.data(), lane->d_labels, stream);
+    if (rc != HPD_OK) {
+        *err = hpd_last_error();
+        return rc;
+    }
+    for (size_t j = 0; j < work.size(); ++j) {
+        const Request& r = *work[j];
+        const size_t bytes = static_cast<size_t>(r.n) * sizeof(int64_t);
+        if (bytes == 0) continue;
+        const int64_t* src = lane->d_labels + offsets[j];
+        e = r.device ? hipMemcpyAsync(r.d_labels, src, bytes, hipMemcpyDeviceToDevice, stream)
+                     : hipMemcpyAsync(lane->h_labels + offsets[j], src, bytes,
+                                      hipMemcpyDeviceToHost, stream);
+        if (e != hipSuccess) return hip_fail(e, "copy of a request's labels");
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return HPD_OK;
+}
+
 void lane_main(hpd_server* srv, Lane* lane) {
     for (;;) {
         std::vector<std::shared_ptr<Request>> work;
@@ -128,17 +245,10 @@ void lane_main(hpd_server* srv, Lane* lane) {
             mps[j] = work[j]->min_pts;
         }
         const int64_t total = offsets.back();
-        std::vector<char> rows(static_cast<size_t>(total) * row_bytes);
-        for (size_t j = 0; j < work.size(); ++j)
-            if (work[j]->n)
-                std::memcpy(rows.data() + static_cast<size_t>(offsets[j]) * row_bytes,
-                            work[j]->rows.data(), work[j]->rows.size());
-        std::vector<int64_t> labels(static_cast<size_t>(total));
         const Clock::time_point t0 = Clock::now();
-        const int rc = hpd_cluster_batch(lane->ctx, dtype, rows.data(), offsets.data(),
-                                         static_cast<int32_t>(work.size()), d, eps.data(),
-                                         mps.data(), labels.data());
-        const std::string err = rc == HPD_OK ? std::string() : std::string(hpd_last_error());
+        std::string err;
+        const int rc = run_batch(srv, lane, work, offsets, eps, mps, dtype, d, row_bytes, total,
+                                 &err);
         const Clock::time_point t1 = Clock::now();
         {
             std::lock_guard<std::mutex> lk(srv->mu);
@@ -149,8 +259,8 @@ void lane_main(hpd_server* srv, Lane* lane) {
                 r.batch_jobs = static_cast<int>(work.size());
                 r.status = rc;
                 r.error = err;
-                if (rc == HPD_OK)
-                    r.labels.assign(labels.begin() + offsets[j], labels.begin() + offsets[j + 1]);
+                if (rc == HPD_OK && !r.device)
+                    r.labels.assign(lane->h_labels + offsets[j], lane->h_labels + offsets[j + 1]);
                 std::vector<char>().swap(r.rows);
                 r.done = true;
                 srv->charged -= r.charge;
@@ -292,27 +402,18 @@ int hpd_server_create(const hpd_server_config* cfg, hpd_server** out) {
     return HPD_OK;
 }
 
-int hpd_server_submit(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
-                      float eps, uint64_t min_pts, uint64_t* ticket) {
-    if (!srv || !ticket) return fail(HPD_EINVAL, "NULL argument");
-    if (dtype_bytes(dtype) == 0) return fail(HPD_EINVAL, "unknown dtype code");
-    if (!(eps > 0.0f)) return fail(HPD_EINVAL, "epsilon needs to be positive");
-    if (n < 0 || d < 1 || (n > 0 && !data)) return fail(HPD_EINVAL, "bad shape or data");
-    auto r = std::make_shared<Request>();
-    r->dtype = dtype;
-    r->d = d;
-    r->n = n;
-    r->eps = eps;
-    r->min_pts = min_pts;
-    r->rows.assign(static_cast<const char*>(data),
-                   static_cast<const char*>(data) + static_cast<size_t>(n) * d * dtype_bytes(dtype));
+}  // extern "C"
+
+namespace {
+
+int enqueue(hpd_server* srv, std::shared_ptr<Request> r, uint64_t* ticket) {
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         if (srv->stop) return fail(HPD_EINVAL, "server is shutting down");
         r->charge = request_charge(*srv, *r);
-        if (r->charge > srv->cfg.hbm_budget_bytes || n > srv->cfg.max_batch_rows)
-            return fail(HPD_ENOMEM, "request of " + std::to_string(n) +
+        if (r->charge > srv->cfg.hbm_budget_bytes || r->n > srv->cfg.max_batch_rows)
+            return fail(HPD_ENOMEM, "request of " + std::to_string(r->n) +
                                         " rows exceeds the server's memory budget");
         r->ticket = srv->next_ticket++;
         srv->table[r->ticket] = r;
@@ -325,6 +426,51 @@ int hpd_server_submit(hpd_server* srv, int dtype, const void* data, int64_t n, i
     }
     srv->cv_sched.notify_one();
     return HPD_OK;
+}
+
+int check_request(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d, float eps,
+                  const uint64_t* ticket) {
+    if (!srv || !ticket) return fail(HPD_EINVAL, "NULL argument");
+    if (dtype_bytes(dtype) == 0) return fail(HPD_EINVAL, "unknown dtype code");
+    if (!(eps > 0.0f)) return fail(HPD_EINVAL, "epsilon needs to be positive");
+    if (n < 0 || d < 1 || (n > 0 && !data)) return fail(HPD_EINVAL, "bad shape or data");
+    return HPD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpd_server_submit(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
+                      float eps, uint64_t min_pts, uint64_t* ticket) {
+    const int rc = check_request(srv, dtype, data, n, d, eps, ticket);
+    if (rc != HPD_OK) return rc;
+    auto r = std::make_shared<Request>();
+    r->dtype = dtype;
+    r->d = d;
+    r->n = n;
+    r->eps = eps;
+    r->min_pts = min_pts;
+    r->rows.assign(static_cast<const char*>(data),
+                   static_cast<const char*>(data) + static_cast<size_t>(n) * d * dtype_bytes(dtype));
+    return enqueue(srv, std::move(r), ticket);
+}
+
+int hpd_server_submit_device(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
+                             float eps, uint64_t min_pts, int64_t* labels_out, uint64_t* ticket) {
+    int rc = check_request(srv, dtype, data, n, d, eps, ticket);
+    if (rc != HPD_OK) return rc;
+    if (n > 0 && !labels_out) return fail(HPD_EINVAL, "labels_out is NULL");
+    auto r = std::make_shared<Request>();
+    r->dtype = dtype;
+    r->d = d;
+    r->n = n;
+    r->eps = eps;
+    r->min_pts = min_pts;
+    r->device = true;
+    r->d_rows = data;
+    r->d_labels = labels_out;
+    return enqueue(srv, std::move(r), ticket);
 }
 
 int hpd_server_wait(hpd_server* srv, uint64_t ticket, int64_t* labels_out, int64_t n,
@@ -349,7 +495,7 @@ int hpd_server_wait(hpd_server* srv, uint64_t ticket, int64_t* labels_out, int64
         info->latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     }
     const int status = r->status;
-    if (status == HPD_OK) {
+    if (status == HPD_OK && !r->device) {
         if (n != r->n || (n > 0 && !labels_out)) return fail(HPD_EINVAL, "labels buffer does not match the request");
         if (n) std::memcpy(labels_out, r->labels.data(), sizeof(int64_t) * static_cast<size_t>(n));
     }
@@ -395,7 +541,11 @@ int hpd_server_destroy(hpd_server* srv) {
     srv->cv_lane.notify_all();
     srv->scheduler.join();
     for (auto& l : srv->lanes) l->thread.join();
-    for (auto& l : srv->lanes) hpd_context_destroy(l->ctx);
+    (void)hipSetDevice(srv->cfg.device);
+    for (auto& l : srv->lanes) {
+        release(*l);
+        hpd_context_destroy(l->ctx);
+    }
     delete srv;
     return HPD_OK;
 }

@@ -220,7 +220,8 @@ int hpd_cluster_batch_device(hpd_context* ctx, int dtype, const void* data,
 /*
  * Continuous-batching server (no counterpart in the reference).
  *
- * hpd_server_submit copies one point cloud and returns a ticket at once.  A
+ * hpd_server_submit copies one point cloud and returns a ticket at once
+ * (hpd_server_submit_device takes one already in device memory).  A
  * scheduler groups waiting requests of the same dtype and dimension into a
  * batch whenever one of `lanes` workers is idle; each worker owns a context
  * (HIP stream + workspace) and runs its batch with hpd_cluster_batch, so the
@@ -265,8 +266,17 @@ void hpd_server_default_config(hpd_server_config* cfg);
 int hpd_server_create(const hpd_server_config* cfg, hpd_server** out);
 int hpd_server_submit(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
                       float eps, uint64_t min_pts, uint64_t* ticket);
-/* Wait for a ticket (timeout_ms < 0: forever) and copy its n labels.
- * HPD_ETIMEOUT leaves the ticket valid; any other return consumes it. */
+/* A request whose rows and labels stay in device memory (of the server's
+ * device): its rows are copied device to device into its batch and its n
+ * labels device to device into labels_out, so nothing crosses PCIe.  Both
+ * buffers must stay valid, and the rows unchanged, until hpd_server_wait
+ * returns for the ticket. */
+int hpd_server_submit_device(hpd_server* srv, int dtype, const void* data, int64_t n, int32_t d,
+                             float eps, uint64_t min_pts, int64_t* labels_out, uint64_t* ticket);
+/* Wait for a ticket (timeout_ms < 0: forever) and copy its n labels (host
+ * requests; a device request's labels are in its own buffer, and labels_out
+ * and n are ignored).  HPD_ETIMEOUT leaves the ticket valid; any other
+ * return consumes it. */
 int hpd_server_wait(hpd_server* srv, uint64_t ticket, int64_t* labels_out, int64_t n,
                     double timeout_ms, hpd_request_info* info);
 int hpd_server_get_stats(hpd_server* srv, hpd_server_stats* out);

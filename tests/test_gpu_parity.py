@@ -544,6 +544,47 @@ def test_server_results_equal_alone_and_in_the_mix(ctx):
         assert_parity(x, e, m, got, oracle.cluster(x, e, m))
 
 
+def test_server_device_and_host_requests_in_one_stream(ctx):
+    """Device-resident requests (rows and labels in HBM) mixed with host
+    requests of the same dimension: every device request's labels land in
+    its own device buffer, equal to clustering it alone, as do the host
+    requests'; the lanes' grow-only buffers serve growing batches."""
+    from hpdbscan_amd import _lib
+    rng = np.random.default_rng(29)
+    reqs = []
+    for i in range(24):
+        n = int(np.exp(rng.uniform(np.log(300), np.log(60000))))
+        x = blobs(n, 3, int(rng.integers(5, 20)), 0.2, 10.0, 500 + i, np.float32)
+        reqs.append((x, float(rng.uniform(0.2, 0.5)), int(rng.integers(4, 33)), i % 2 == 0))
+    say(f"server: {len(reqs)} requests, half device-resident")
+    bufs = []
+    try:
+        with _lib.Server(lanes=2) as srv:
+            tickets = []
+            for x, e, m, dev in reqs:
+                if dev:
+                    xd = _lib.DeviceBuffer.from_array(ctx, x)
+                    ld = _lib.DeviceBuffer(ctx, 8 * len(x))
+                    bufs.append((xd, ld))
+                    tickets.append(srv.submit_device(xd.ptr, len(x), 3, np.float32, e, m, ld.ptr))
+                else:
+                    bufs.append(None)
+                    tickets.append(srv.submit(x, e, m))
+            outs = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        for (x, e, m, dev), (got, info), b in zip(reqs, outs, bufs):
+            if dev:
+                assert got is None
+                got = b[1].to_array(np.int64, len(x))
+            np.testing.assert_array_equal(got, ctx.cluster(x, e, m))
+    finally:
+        for b in bufs:
+            if b:
+                b[0].close()
+                b[1].close()
+
+
 def test_server_admission_rejects_oversize(ctx):
     from hpdbscan_amd import _lib
     say("server admission")
