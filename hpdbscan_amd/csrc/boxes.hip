@@ -29,6 +29,7 @@
 #include <climits>
 
 #include "boxes.hpp"
+#include "hpdbscan_capi.h"
 #include "common.hpp"
 #include "grid.hpp"
 #include "index.hpp"
@@ -298,11 +299,11 @@ __global__ __launch_bounds__(kThreads) void k_box_hook(
     for (int32_t wid = uniform(static_cast<int32_t>(blockIdx.x) * kWavesPerBlock +
                                static_cast<int32_t>(threadIdx.x >> 6));
          wid < nitems; wid += stride) [&] {
-        // boxes with cores; one hook per box is enough: its first chunk looks
-        // for it (a missed link is found by the exact union later)
+        // boxes with cores of more than kSmallBox rows; one hook per box is
+        // enough: its first chunk looks for it (a missed link is found by
+        // the exact union later)
         const int32_t b = uniform(blist[wid]);
         const int2 it = make_int2(b, uniform(bstart[b]));
-        if (uniform(bstart[b + 1]) - it.y <= kSmallBox) return;  // k_box_hook_small
         const int32_t end = uniform(bstart[b + 1]);
         const int32_t p = it.y + lane_id();
         const bool pcore = p < end && core[p];
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void k_box_hook_small(
     if (i >= nlist) return;
     const int32_t b = blist[i];
     const int32_t s0 = bstart[b], s1 = bstart[b + 1];
-    if (s1 - s0 > kSmallBox) return;  // one wave per box instead
+    if (s1 - s0 > kSmallBox) return;  // k_box_hook's
     const int32_t u = bcell[b];
     const JobParams<T>& jp = jobs[job_of_cell[u]];
     const T eps2 = jp.eps2;
@@ -740,11 +741,15 @@ __device__ __forceinline__ void block_select(int64_t n, unsigned int* count, Tak
 __global__ __launch_bounds__(kThreads) void k_select_boxes(Count nbox_c, int mode,
                                                            const int32_t* __restrict__ v,
                                                            int32_t* __restrict__ list,
-                                                           unsigned int* __restrict__ count) {
+                                                           unsigned int* __restrict__ count,
+                                                           const int32_t* __restrict__ bstart) {
     const int32_t nbox = static_cast<int32_t>(nbox_c.get());
     block_select(
         nbox, count,
-        [&](int64_t b) { return mode == 0 ? v[b] != INT_MAX : v[b] == static_cast<int32_t>(b); },
+        [&](int64_t b) {
+            if (mode == 1) return v[b] == static_cast<int32_t>(b);
+            return v[b] != INT_MAX && (mode == 0 || bstart[b + 1] - bstart[b] > kSmallBox);
+        },
         [&](int64_t b, unsigned int pos) { list[pos] = static_cast<int32_t>(b); });
 }
 
@@ -1109,14 +1114,18 @@ inline unsigned wave_blocks(Count items) {
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
-                     Count nlist, int32_t* bparent, hipStream_t s) {
+                     Count nlist, int32_t* bparent, int32_t* big, unsigned int* nbig,
+                     hipStream_t s) {
     using U = OrdKey<T>;
     if (a.nbox > 0)
         hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(a.nbox, kThreads)), dim3(kThreads), 0, s,
                            bparent, static_cast<int64_t>(a.nbox));
     if (nlist.host > 0) {
-        hipLaunchKernelGGL((k_box_hook<T, D>), dim3(wave_blocks(nlist)), dim3(kThreads), 0, s, a.soa,
-                           a.ld, blist, nlist, a.bstart, a.bcell, a.cbox, roff, rcells,
+        // the larger boxes one wave each (first: their hooks are found
+        // soonest), the small ones one thread each
+        const Count nb{nlist.host, nbig};
+        hipLaunchKernelGGL((k_box_hook<T, D>), dim3(wave_blocks(nb)), dim3(kThreads), 0, s, a.soa,
+                           a.ld, big, nb, a.bstart, a.bcell, a.cbox, roff, rcells,
                            reinterpret_cast<const U*>(a.bbox), bmincore, core, a.job_of_cell, a.jobs,
                            a.rcenter, bparent);
         hipLaunchKernelGGL((k_box_hook_small<T, D>), dim3(blocks_for(nlist.host, kThreads)),
@@ -1146,10 +1155,11 @@ void launch_box_root_join(const BoxArgs<T>& a, const uint32_t* roff, const int2*
 }
 
 void launch_select_boxes(Count nbox, int mode, const int32_t* v, int32_t* list,
-                         unsigned int* count, hipStream_t s) {
+                         unsigned int* count, hipStream_t s, const int32_t* bstart) {
     if (nbox.host <= 0) return;
+    if (mode == 2 && !bstart) throw Error(HPD_EINVAL, "launch_select_boxes: mode 2 needs bstart");
     hipLaunchKernelGGL(k_select_boxes, dim3(blocks_for(nbox.host, kSelectSpan)), dim3(kThreads), 0,
-                       s, nbox, mode, v, list, count);
+                       s, nbox, mode, v, list, count, bstart);
 }
 
 void launch_core_full(int64_t n, const int32_t* box_of_slot, const uint8_t* btight,
@@ -1291,7 +1301,7 @@ void launch_positions(int64_t n, int32_t m, const int32_t* box_of_slot, const in
                                          hipStream_t);                                            \
     template void launch_box_hook<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,          \
                                         const int32_t*, const uint8_t*, const int32_t*, Count,    \
-                                        int32_t*, hipStream_t);                                   \
+                                        int32_t*, int32_t*, unsigned int*, hipStream_t);          \
     template void launch_box_root_join<T, D>(const BoxArgs<T>&, const uint32_t*, const int2*,     \
                                              const int32_t*, const uint8_t*, const int32_t*,      \
                                              Count, int32_t*, hipStream_t);                       \

@@ -66,11 +66,15 @@ template <typename T, int D>
 void launch_box_tight(const BoxArgs<T>& a, uint8_t* btight, unsigned int* nloose, hipStream_t s);
 
 // Components of the boxes with cores:
-//   launch_select_boxes  list of the boxes with cores (mode 0, v = bmincore)
-//                        or of the roots (mode 1, v = rsnap); *count += n
+//   launch_select_boxes  list of the boxes with cores (mode 0, v = bmincore),
+//                        of those with more than kSmallBox rows (mode 2,
+//                        with bstart) or of the roots (mode 1, v = rsnap);
+//                        *count += n
 //   launch_box_hook      hook forest over the listed boxes (bparent is
 //                        reset first): every box points at some earlier box
-//                        joined to it
+//                        joined to it; big[0, *nbig) lists the boxes of
+//                        more than kSmallBox rows (launch_select_boxes mode
+//                        2), which take one wave each
 //   launch_box_roots     root of every box -> rsnap (-1 without cores)
 //   launch_box_root_join every listed root joins the first tree of another
 //                        box it meets (kRootJoinRounds rounds, each followed
@@ -80,11 +84,12 @@ template <int D>
 constexpr int kRootJoinRounds = D <= 3 ? 1 : 2;
 
 void launch_select_boxes(Count nbox, int mode, const int32_t* v, int32_t* list,
-                         unsigned int* count, hipStream_t s);
+                         unsigned int* count, hipStream_t s, const int32_t* bstart = nullptr);
 template <typename T, int D>
 void launch_box_hook(const BoxArgs<T>& a, const uint32_t* roff, const int2* rcells,
                      const int32_t* bmincore, const uint8_t* core, const int32_t* blist,
-                     Count nlist, int32_t* bparent, hipStream_t s);
+                     Count nlist, int32_t* bparent, int32_t* big, unsigned int* nbig,
+                     hipStream_t s);
 void launch_box_roots(Count nbox, const int32_t* bmincore, int32_t* bparent, int32_t* rsnap,
                       hipStream_t s);
 template <typename T, int D>

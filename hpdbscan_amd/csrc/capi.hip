@@ -645,8 +645,11 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         // compmin holds the forest's roots until the clusters are known
         tm.mark(M_HOOK0);
         launch_select_boxes(host_count(nbox), 0, bmincore, blist, &counters[8], s);
+        // (the slot-level union's rsnap is free in box mode: the list of
+        // the larger boxes)
+        launch_select_boxes(host_count(nbox), 2, bmincore, rsnap, &counters[14], s, bstart);
         launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist,
-                              Count{nbox, &counters[8], true}, bparent, s);
+                              Count{nbox, &counters[8], true}, bparent, rsnap, &counters[14], s);
         launch_box_roots(host_count(nbox), bmincore, bparent, compmin, s);
         for (int round = 0; round < kRootJoinRounds<D>; ++round) {
             HPD_HIP(hipMemsetAsync(&counters[9], 0, sizeof(uint32_t), s));
@@ -802,6 +805,7 @@ enum StaticSlot : int {
     SS_GRID = 15,      // grid beyond 64 key bits
     SS_BITS = 16,      // key bits + sub-box bits
     SS_ROWTAB = 17,    // the row-order label table holds every key
+    SS_HOOKBIG = 18,   // boxes with cores of more than 32 rows
     SS_WORDS = 32
 };
 
@@ -877,7 +881,7 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     uint32_t* nchunks = A.take_n<uint32_t>(nn + 1);
     uint8_t* core = A.take_n<uint8_t>(L);
     int32_t* parent = A.take_n<int32_t>(L);
-    (void)A.take_n<int32_t>(L);  // rsnap of the slot-level union (not used here)
+    int32_t* hook_big = A.take_n<int32_t>(L);  // (rsnap of the slot-level union)
     int32_t* rootslot = A.take_n<int32_t>(L);
     int32_t* cmin = A.take_n<int32_t>(nn);
     int32_t* cmax = A.take_n<int32_t>(nn);
@@ -1011,8 +1015,9 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     // the box union (its result is discarded when a box is loose)
     int32_t* blist = parent;
     launch_select_boxes(NB, 0, bmincore, blist, &sc[SS_NLIST], s);
+    launch_select_boxes(NB, 2, bmincore, hook_big, &sc[SS_HOOKBIG], s, bstart);
     launch_box_hook<T, D>(ba, nranges, rcells, bmincore, core, blist, Count{n, &sc[SS_NLIST]},
-                          bparent, s);
+                          bparent, hook_big, &sc[SS_HOOKBIG], s);
     launch_box_roots(NB, bmincore, bparent, compmin, s);
     for (int round = 0; round < kRootJoinRounds<D>; ++round) {
         uint32_t* cnt = &sc[round == 0 ? SS_JOIN0 : SS_JOIN1];
