@@ -57,9 +57,9 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 # The kernels that compare rows with their neighbours, keyed by the
 # per-kernel time the library reports (HIP events on its stream).
 PAIR_KERNELS = {
-    "ms_core": "k_core_boxes",
-    "ms_hook_kernel": "k_box_hook",
-    "ms_union_kernel": "k_box_union",
+    "ms_core_kernel": "k_core_boxes",
+    "ms_hook_kernel": "k_box_hook+k_box_hook_small",
+    "ms_union_kernel": "k_box_scan+k_box_union_list+k_box_union",
     "ms_border_kernel": "k_border_boxes+k_border_sparse",
 }
 
@@ -73,10 +73,12 @@ def compulsory_bytes(kernel: str, st: dict, n: int, d: int, s: int) -> int:
                       per box first row 4, bounding box 2*D*s, flags 1;
                       per cell first row, ranges offset, own row, job (16);
                       per neighbour row its cells (8)
-      k_box_hook      rows n*D*s, core flag n; per box first row, cell,
-                      smallest core, parent written (16) + bounding box;
-                      per cell ranges offset, first box, job (12); 8 per row
-      k_box_union     as k_box_hook plus the hook roots (4 per box)
+      k_box_hook*     (the hook forest) rows n*D*s, core flag n; per box
+                      first row, cell, smallest core, parent written (16) +
+                      bounding box; per cell ranges offset, first box, job
+                      (12); 8 per row
+      k_box_scan*     (the open-pair union) as the hook forest plus the
+                      hook roots (4 per box)
       k_border_*      rows n*D*s, core flag n, input row 4n, box of row 4n,
                       labels written 8 per non-core row; per box first row,
                       cluster, cell (12) + bounding box; per cell 16; 8 per
@@ -86,9 +88,9 @@ def compulsory_bytes(kernel: str, st: dict, n: int, d: int, s: int) -> int:
     noncore = n - int(st["core_points"])
     if kernel == "k_core_boxes":
         return n * (d * s + 5) + nb * (2 * d * s + 5) + m * 16 + r * 8
-    if kernel == "k_box_hook":
+    if kernel.startswith("k_box_hook"):
         return n * (d * s + 1) + nb * (2 * d * s + 16) + m * 12 + r * 8
-    if kernel == "k_box_union":
+    if kernel.startswith("k_box_scan"):
         return n * (d * s + 1) + nb * (2 * d * s + 20) + m * 12 + r * 8
     return n * (d * s + 9) + 8 * noncore + nb * (2 * d * s + 12) + m * 16 + r * 8
 
@@ -623,7 +625,7 @@ def main() -> None:
           file=sys.stderr, flush=True)
     st = phases[-1]
     keys = ["ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union", "ms_border", "ms_total",
-            "ms_hook_kernel", "ms_union_kernel", "ms_border_kernel"]
+            "ms_core_kernel", "ms_hook_kernel", "ms_union_kernel", "ms_border_kernel"]
     med = {k: float(np.median([p[k] for p in phases])) for k in keys}
     roofline = kernel_roofline(st, med, n, d, x.dtype.itemsize, a.workload)
     pair_ms = med["ms_core"] + med["ms_hook_kernel"] + med["ms_union_kernel"] + med["ms_border_kernel"]

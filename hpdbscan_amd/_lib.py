@@ -60,6 +60,7 @@ class RunStats(ctypes.Structure):
         ("box_union", ctypes.c_int32),
         ("neighbor_rows", ctypes.c_uint64),
         ("open_box_pairs", ctypes.c_uint64),
+        ("ms_core_kernel", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

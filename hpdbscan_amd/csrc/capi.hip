@@ -268,6 +268,7 @@ enum Mark : int {
     M_GRID,        // min/max and cell ids done
     M_SORT,        // both sorts and the gather done
     M_INDEX,       // cells, neighbour rows, boxes done
+    M_CORE0,       // the core count kernel starts (after the full boxes' flags)
     M_CORE,        // core flags done
     M_HOOK0,       // box union: hook forest starts
     M_HOOK1,       // box union: hook forest and its roots done
@@ -282,14 +283,19 @@ class PhaseTimer {
 public:
     PhaseTimer(Context& c, hipStream_t s) : ctx_(c), s_(s) {}
     void mark(Mark m) {
-        if (!ctx_.timing) return;
+        hipEvent_t e = event(m);
+        if (e) HPD_HIP(hipEventRecord(e, s_));
+    }
+    // the event of mark m for the caller to record (null when timing is off)
+    hipEvent_t event(Mark m) {
+        if (!ctx_.timing) return nullptr;
         while (ctx_.events.size() < M_COUNT) {
             hipEvent_t e;
             HPD_HIP(hipEventCreate(&e));
             ctx_.events.push_back(e);
         }
-        HPD_HIP(hipEventRecord(ctx_.events[m], s_));
         seen_[m] = true;
+        return ctx_.events[m];
     }
     // milliseconds between two recorded marks (after a stream sync); 0 if
     // either was not recorded in this run
@@ -633,7 +639,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     }
     tm.mark(M_INDEX);
 
-    launch_core<T, D>(a, core, &dstats[1], s);
+    launch_core<T, D>(a, core, &dstats[1], s, tm.event(M_CORE0));
     tm.mark(M_CORE);
 
     // box lists of the union (box mode leaves the slot-level parent free)
@@ -1293,6 +1299,7 @@ static void run_batch(Context& ctx, const T* pts, int64_t n, const int64_t* offs
         ctx.stats.ms_hook_kernel = tm.ms(M_HOOK0, M_HOOK1);
         ctx.stats.ms_union_kernel = tm.ms(M_HOOK1, M_JOIN1);
         ctx.stats.ms_border_kernel = tm.ms(M_BORDER0, M_LABELS);
+        ctx.stats.ms_core_kernel = tm.ms(M_CORE0, M_CORE);
     }
 }
 

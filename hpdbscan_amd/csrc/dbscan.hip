@@ -1120,7 +1120,9 @@ void count_noise(const int64_t* labels, int64_t n, unsigned long long* out, hipS
 }
 
 template <typename T, int D>
-void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s) {
+void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s,
+                 hipEvent_t count_start) {
+    if (count_start) HPD_HIP(hipEventRecord(count_start, s));  // (no box pass: all of it)
     if (a.nitems <= 0) return;
     if constexpr (std::is_floating_point<T>::value) {
         if (a.bbox) {
@@ -1146,6 +1148,7 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
                 HPD_HIP(hipMemsetAsync(core, 0, static_cast<size_t>(a.n), s));
             }
             const ItemGrid g = item_grid(ac.nditems, ac.nditems_dev != nullptr, a.n);
+            if (count_start) HPD_HIP(hipEventRecord(count_start, s));
             if (g.waves > 0)
                 hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(g.waves, kWavesPerBlock)),
                                    dim3(kThreads), 0, s, ac, core, ncore);
@@ -1294,7 +1297,7 @@ void launch_one_thread_assign(const DbscanArgs<T>& a, const uint8_t* core, const
 
 #define HPD_INST_DBSCAN(T, D)                                                                     \
     template void launch_core<T, D>(const DbscanArgs<T>&, uint8_t*, unsigned long long*,          \
-                                    hipStream_t);                                                 \
+                                    hipStream_t, hipEvent_t);                                     \
     template void launch_union_points<T, D>(const DbscanArgs<T>&, const uint8_t*, int32_t*,       \
                                             int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t); \
     template void launch_border<T, D>(const DbscanArgs<T>&, const uint8_t*, const int32_t*,       \

@@ -94,6 +94,7 @@ typedef struct hpd_run_stats {
     int32_t box_union;           /* 1 if the union ran on sub-boxes */
     uint64_t neighbor_rows;      /* [first, end) slot ranges of all cells */
     uint64_t open_box_pairs;     /* box pairs the hook forest left open */
+    double ms_core_kernel;       /* the core count kernel alone (part of ms_core) */
 } hpd_run_stats;
 
 /* Thread-local message describing the last failure ("" if none). */
