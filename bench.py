@@ -129,6 +129,26 @@ def kernel_roofline(st: dict, med: dict, n: int, d: int, s: int, workload: str) 
                     "kernel's time goes to dependent loads and issue, not to bandwidth"}
 
 
+# MI355X FP32 vector rate: 157.3 TFLOP/s counting an FMA as 2, so 78.6 T
+# lane-ops/s of plain adds / multiplies / compares
+PEAK_VALU_LANE_OPS = 78.6e12
+
+
+def core_test_figures(st: dict, med: dict, d: int) -> dict:
+    """The row-row distance tests the core count kernel executed (the rest
+    of the reference's candidate tests are decided by bounding boxes), and
+    their VALU rate: 3D lane-ops per test (D subtracts, D multiplies, D - 1
+    adds, 1 compare) over the kernel's event-timed duration."""
+    tests = int(st["core_tests"])
+    ms = med["ms_core_kernel"]
+    ops = tests * 3 * d
+    rate = ops / (ms * 1e-3) if ms > 0 else 0.0
+    return {"executed": tests,
+            "fraction_of_sum_candidates": round(tests / max(1, int(st["sum_candidates"])), 6),
+            "lane_ops_per_s": round(rate, 1), "peak_lane_ops_per_s": PEAK_VALU_LANE_OPS,
+            "valu_frac": round(rate / PEAK_VALU_LANE_OPS, 6), "kernel_ms": round(ms, 3)}
+
+
 def cpu_threads() -> int:
     """Every host CPU this process may run on."""
     try:
@@ -627,6 +647,12 @@ def main() -> None:
     keys = ["ms_grid", "ms_sort", "ms_index", "ms_core", "ms_union", "ms_border", "ms_total",
             "ms_core_kernel", "ms_hook_kernel", "ms_union_kernel", "ms_border_kernel"]
     med = {k: float(np.median([p[k] for p in phases])) for k in keys}
+    # one more (untimed) clustering with the instrumented count kernel: the
+    # distance tests it executes
+    ctx.set_timing(2)
+    ctx.cluster(x, eps, mp)
+    st_tests = ctx.stats()
+    ctx.set_timing(False)
     roofline = kernel_roofline(st, med, n, d, x.dtype.itemsize, a.workload)
     pair_ms = med["ms_core"] + med["ms_hook_kernel"] + med["ms_union_kernel"] + med["ms_border_kernel"]
 
@@ -653,6 +679,7 @@ def main() -> None:
             "note": "the reference's distance tests (3^D candidate cells per row) divided by the "
                     "summed time of the core, hook, union and border kernels, which skip tests "
                     "that bounding boxes decide"},
+        "core_tests": core_test_figures({**st, "core_tests": st_tests["core_tests"]}, med, d),
         "stats": {k: int(st[k]) for k in ("occupied_cells", "core_points", "noise_points",
                                           "ambiguous_borders", "sum_candidates", "boxes",
                                           "neighbor_rows", "open_box_pairs")},

@@ -61,6 +61,7 @@ class RunStats(ctypes.Structure):
         ("neighbor_rows", ctypes.c_uint64),
         ("open_box_pairs", ctypes.c_uint64),
         ("ms_core_kernel", ctypes.c_double),
+        ("core_tests", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -249,8 +250,10 @@ class Context:
         check(load().hpd_context_get_work_split(self._h, ctypes.byref(sm), ctypes.byref(sc)))
         return int(sm.value), bool(sc.value)
 
-    def set_timing(self, enabled: bool) -> None:
-        check(load().hpd_context_set_timing(self._h, 1 if enabled else 0))
+    def set_timing(self, enabled) -> None:
+        """True / 1: phase events; 2: also count the core count's executed
+        distance tests (an instrumented, slower kernel)."""
+        check(load().hpd_context_set_timing(self._h, int(enabled)))
 
     def set_graph(self, enabled: bool) -> None:
         """Graph mode: floating-point runs take the static pipeline, captured

@@ -229,6 +229,7 @@ struct Context {
     uint64_t open_pair_limit = 0;  // 0: 2n
     bool sparse_core = false;
     bool timing = false;
+    bool count_tests = false;  // hpd_context_set_timing level 2
     hpd_run_stats stats{};
     uint64_t* host_scratch = nullptr;  // pinned, 64 words
     std::vector<hipEvent_t> events;
@@ -238,6 +239,7 @@ struct Context {
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
         int dtype = -1, d = 0, border_rule = -1, sparse_max = -1, box_split = -1;
+        bool count_tests = false;
         bool sparse_core = false;
         int64_t n = 0;
         const void* data = nullptr;
@@ -639,7 +641,7 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     }
     tm.mark(M_INDEX);
 
-    launch_core<T, D>(a, core, &dstats[1], s, tm.event(M_CORE0));
+    launch_core<T, D>(a, core, ctx.count_tests ? &dstats[7] : nullptr, s, tm.event(M_CORE0));
     tm.mark(M_CORE);
 
     // box lists of the union (box mode leaves the slot-level parent free)
@@ -724,9 +726,10 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
                         &dstats[2], amb, noise_stripes, s);
 
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx.host_scratch);
-    HPD_HIP(hipMemcpyAsync(hs, dstats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HPD_HIP(hipMemcpyAsync(hs, dstats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HPD_HIP(hipStreamSynchronize(s));
     if (box_union) ctx.stats.open_box_pairs = hs[5];
+    ctx.stats.core_tests = hs[7];
     if (one_thread && hs[3] > 0) {
         // the reference's choice at one thread for the border rows next to
         // two or more clusters (hpdbscan.h:74-92)
@@ -1016,7 +1019,7 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
     a.sitems = sitems;
     a.nsitems = static_cast<int32_t>(n);
     a.nsitems_dev = &sc[SS_SITEMS];
-    launch_core<T, D>(a, core, &dstats[1], s);
+    launch_core<T, D>(a, core, ctx.count_tests ? &dstats[7] : nullptr, s);
 
     // the box union (its result is discarded when a box is loose)
     int32_t* blist = parent;
@@ -1119,7 +1122,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
                       ge.labels == labels && ge.arena_main == A.base() &&
                       ge.arena_aux == ctx.aux.base() && ge.border_rule == ctx.border_rule &&
                       ge.sparse_max == ctx.sparse_max && ge.sparse_core == ctx.sparse_core &&
-                      ge.box_split == ctx.box_split &&
+                      ge.box_split == ctx.box_split && ge.count_tests == ctx.count_tests &&
                       ge.offsets.size() == static_cast<size_t>(njobs) + 1 &&
                       std::equal(ge.offsets.begin(), ge.offsets.end(), offsets);
     if (!same) {
@@ -1151,6 +1154,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
         ge.border_rule = ctx.border_rule;
         ge.sparse_max = ctx.sparse_max;
         ge.box_split = ctx.box_split;
+        ge.count_tests = ctx.count_tests;
         ge.sparse_core = ctx.sparse_core;
         ge.offsets.assign(offsets, offsets + njobs + 1);
         ++ctx.graph_captures;
@@ -1175,6 +1179,7 @@ static bool run_static(Context& ctx, int dtype, const T* pts, int64_t n, const i
     ctx.stats.noise_points = hs[2];
     ctx.stats.ambiguous_borders = hs[3];
     ctx.stats.open_box_pairs = hs[5];
+    ctx.stats.core_tests = hs[7];
     ctx.stats.total_cells = hs[6];
     ctx.stats.key_bits = static_cast<int32_t>(hc[SS_BITS]) - box_sub_bits(D, split_of(ctx, D));
     ctx.stats.occupied_cells = hc[SS_M];
@@ -1554,6 +1559,7 @@ int hpd_context_set_timing(hpd_context* ctx, int enabled) {
     return guarded([&] {
         if (!ctx) throw Error(HPD_EINVAL, "context is NULL");
         ctx->impl.timing = enabled != 0;
+        ctx->impl.count_tests = enabled >= 2;
     });
 }
 

@@ -178,12 +178,16 @@ __global__ __launch_bounds__(kThreads) void k_core(const DbscanArgs<T> a, uint8_
 // and gmin2 <= d^2 <= gmax2, the reference's float expression is within a
 // few ulps of d^2, so gmin2 > EPS2 (1 + 1e-5) (+ 1e-30) rules every member
 // out and gmax2 <= EPS2 (1 - 1e-5) rules every member in.
-template <typename T, int D>
+// kCount: also add the row-row distance tests executed to *ntests (an
+// instrumented variant: the count is one more live value in a kernel that
+// has no VGPR to spare at 8 waves per SIMD)
+template <typename T, int D, bool kCount = false>
 __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t* __restrict__ core,
-                                                unsigned long long* __restrict__ ncore,
+                                                unsigned long long* __restrict__ ntests,
                                                 int32_t wid) {
     Chunk c;
     chunk_at(a.ditems, wid, a.ustart, a.roff, c);
+    uint32_t tests = 0;  // row-row distance tests executed (lanes x rows visited)
     // the first group's neighbour boxes are loaded ahead, beside the rows
     // and their boxes (the chunk was selected because a row needs counting)
     const int32_t center = uniform(a.rcenter[c.u]);
@@ -259,8 +263,10 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
                         const int j = static_cast<int>(__builtin_ctzll(todo));
                         const bool part = need && ((bcast64(part_rows, j) >> lane_id()) & 1);
                         if (__any(part)) {
+                            const uint32_t lanes = static_cast<uint32_t>(__popcll(ballot(part)));
                             auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
                                 cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
+                                if constexpr (kCount) tests += lanes;
                             };
                             auto stop = [&] {
                                 return static_cast<bool>(__all(!part || cnt >= min_pts));
@@ -289,8 +295,10 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
                     if (whole) cnt += static_cast<uint32_t>(q1 - q0);
                     const bool part = near && !whole;
                     if (__any(part)) {
+                        const uint32_t lanes = static_cast<uint32_t>(__popcll(ballot(part)));
                         auto body = [&](int32_t, const T* qc, uint8_t, int32_t) {
                             cnt += (part && within<D>(pc, qc, eps2)) ? 1u : 0u;
+                            if constexpr (kCount) tests += lanes;
                         };
                         auto stop = [&] { return static_cast<bool>(__all(!part || cnt >= min_pts)); };
                         visit_range<T, D>(a.soa, a.ld, nullptr, nullptr, q0, q1, body, stop);
@@ -303,7 +311,8 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
     }
     const bool is_core = valid && cnt >= min_pts;
     if (valid) core[p] = is_core ? 1 : 0;
-    (void)ncore;
+    if constexpr (kCount)
+        if (tests != 0 && lane_id() == 0) atomicAdd(ntests, static_cast<unsigned long long>(tests));
     if (a.bmincore) {
         // smallest core input row per box.  A box holds its rows in input
         // order, and every row of a tight box with >= minPts rows is core:
@@ -326,25 +335,26 @@ __device__ __forceinline__ void core_boxes_item(const DbscanArgs<T>& a, uint8_t*
 }
 
 // One wave per work item.
-template <typename T, int D>
+template <typename T, int D, bool kCount = false>
 __global__ __launch_bounds__(kThreads, 8) void k_core_boxes(const DbscanArgs<T> a,
                                                          uint8_t* __restrict__ core,
-                                                         unsigned long long* __restrict__ ncore) {
+                                                         unsigned long long* __restrict__ ntests) {
     const int32_t wid = first_wave();
-    if (wid < item_count(a.nditems, a.nditems_dev)) core_boxes_item<T, D>(a, core, ncore, wid);
+    if (wid < item_count(a.nditems, a.nditems_dev))
+        core_boxes_item<T, D, kCount>(a, core, ntests, wid);
 }
 
 // The waves walk items wid0, wid0 + 1, ...: the items of a device-written
 // count beyond what a one-wave-per-item grid covered.  The walk keeps the
 // arguments live across items, more than the 64 VGPRs of 8 waves per SIMD
 // hold without spilling, so it runs at 4.
-template <typename T, int D>
+template <typename T, int D, bool kCount = false>
 __global__ __launch_bounds__(kThreads, 4) void k_core_boxes_walk(
-    const DbscanArgs<T> a, uint8_t* __restrict__ core, unsigned long long* __restrict__ ncore,
+    const DbscanArgs<T> a, uint8_t* __restrict__ core, unsigned long long* __restrict__ ntests,
     int32_t wid0) {
     const int32_t nwork = item_count(a.nditems, a.nditems_dev);
     for (int32_t wid = wid0 + first_wave(); wid < nwork; wid += wave_stride())
-        core_boxes_item<T, D>(a, core, ncore, wid);
+        core_boxes_item<T, D, kCount>(a, core, ntests, wid);
 }
 
 // Thread-per-row core counts for the rows of sparse cells (a.sitems): the
@@ -1120,8 +1130,9 @@ void count_noise(const int64_t* labels, int64_t n, unsigned long long* out, hipS
 }
 
 template <typename T, int D>
-void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s,
+void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ntests, hipStream_t s,
                  hipEvent_t count_start) {
+    unsigned long long* ncore = nullptr;  // (the other core kernels count nothing)
     if (count_start) HPD_HIP(hipEventRecord(count_start, s));  // (no box pass: all of it)
     if (a.nitems <= 0) return;
     if constexpr (std::is_floating_point<T>::value) {
@@ -1149,13 +1160,26 @@ void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncor
             }
             const ItemGrid g = item_grid(ac.nditems, ac.nditems_dev != nullptr, a.n);
             if (count_start) HPD_HIP(hipEventRecord(count_start, s));
-            if (g.waves > 0)
-                hipLaunchKernelGGL((k_core_boxes<T, D>), dim3(blocks_for(g.waves, kWavesPerBlock)),
-                                   dim3(kThreads), 0, s, ac, core, ncore);
-            if (g.tail)
-                hipLaunchKernelGGL((k_core_boxes_walk<T, D>),
-                                   dim3(blocks_for(kTailWaves, kWavesPerBlock)), dim3(kThreads), 0,
-                                   s, ac, core, ncore, static_cast<int32_t>(g.waves));
+            if (g.waves > 0) {
+                if (ntests)
+                    hipLaunchKernelGGL((k_core_boxes<T, D, true>),
+                                       dim3(blocks_for(g.waves, kWavesPerBlock)), dim3(kThreads), 0,
+                                       s, ac, core, ntests);
+                else
+                    hipLaunchKernelGGL((k_core_boxes<T, D>),
+                                       dim3(blocks_for(g.waves, kWavesPerBlock)), dim3(kThreads), 0,
+                                       s, ac, core, nullptr);
+            }
+            if (g.tail) {
+                if (ntests)
+                    hipLaunchKernelGGL((k_core_boxes_walk<T, D, true>),
+                                       dim3(blocks_for(kTailWaves, kWavesPerBlock)), dim3(kThreads),
+                                       0, s, ac, core, ntests, static_cast<int32_t>(g.waves));
+                else
+                    hipLaunchKernelGGL((k_core_boxes_walk<T, D>),
+                                       dim3(blocks_for(kTailWaves, kWavesPerBlock)), dim3(kThreads),
+                                       0, s, ac, core, nullptr, static_cast<int32_t>(g.waves));
+            }
             if (ac.nsitems > 0)
                 hipLaunchKernelGGL((k_core_sparse<T, D>), dim3(blocks_for(ac.nsitems, kThreads)),
                                    dim3(kThreads), 0, s, ac, core, ncore);

@@ -68,8 +68,10 @@ struct DbscanArgs {
 // by box (boxes beyond eps skipped, boxes within eps counted whole).  The
 // number of cores is added to *ncore by launch_label_cores.
 template <typename T, int D>
-// (count_start, when not null, is recorded right before the count kernel)
-void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ncore, hipStream_t s,
+// (count_start, when not null, is recorded right before the count kernel;
+// with ntests not null the box path runs an instrumented count kernel that
+// adds the row-row distance tests it executes to *ntests)
+void launch_core(const DbscanArgs<T>& a, uint8_t* core, unsigned long long* ntests, hipStream_t s,
                  hipEvent_t count_start = nullptr);
 
 // Components of the core-core graph on slots (used when boxes are not):

@@ -95,6 +95,9 @@ typedef struct hpd_run_stats {
     uint64_t neighbor_rows;      /* [first, end) slot ranges of all cells */
     uint64_t open_box_pairs;     /* box pairs the hook forest left open */
     double ms_core_kernel;       /* the core count kernel alone (part of ms_core) */
+    uint64_t core_tests;         /* row-row distance tests the core count executed
+                                    (box mode, timing level 2; bounding boxes
+                                    decide the rest) */
 } hpd_run_stats;
 
 /* Thread-local message describing the last failure ("" if none). */
@@ -114,6 +117,9 @@ int hpd_context_destroy(hpd_context* ctx);
 
 /* Options. */
 int hpd_context_set_border_rule(hpd_context* ctx, int rule);
+/* Phase timing: 1 records HIP events between the phases (ms_* of the
+ * statistics); 2 also runs an instrumented core count kernel that counts
+ * the distance tests it executes (core_tests; slower, for measurement). */
 int hpd_context_set_timing(hpd_context* ctx, int enabled);
 int hpd_context_get_stats(const hpd_context* ctx, hpd_run_stats* out);
 

@@ -181,6 +181,16 @@ def test_box_union_used_and_exact(ctx):
     st = ctx.stats()
     assert st["box_union"] == 1 and st["boxes"] > 0
     assert_parity(x, 0.3, 8, got, oracle.cluster(x, 0.3, 8))
+    # the instrumented core count (timing level 2): same labels; it executes
+    # some of the reference's candidate tests, the bounding boxes decide the
+    # others
+    ctx.set_timing(2)
+    try:
+        np.testing.assert_array_equal(ctx.cluster(x, 0.3, 8), got)
+        st2 = ctx.stats()
+    finally:
+        ctx.set_timing(True)
+    assert 0 < st2["core_tests"] < st2["sum_candidates"]
 
 
 @pytest.mark.parametrize("d", [2, 3])
