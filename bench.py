@@ -621,6 +621,9 @@ def main() -> None:
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--no-core-tests", dest="core_tests", action="store_false",
+                    help="skip the instrumented clustering that counts the core count's "
+                         "executed distance tests (keeps kernel traces to the timed runs)")
     ap.add_argument("--serve-sweep", type=float, nargs="*", default=[2.0, 4.0],
                     help="serving: also replay the stream at these multiples of its rate")
     a = ap.parse_args()
@@ -649,10 +652,12 @@ def main() -> None:
     med = {k: float(np.median([p[k] for p in phases])) for k in keys}
     # one more (untimed) clustering with the instrumented count kernel: the
     # distance tests it executes
-    ctx.set_timing(2)
-    ctx.cluster(x, eps, mp)
-    st_tests = ctx.stats()
-    ctx.set_timing(False)
+    st_tests = {"core_tests": 0}
+    if a.core_tests:
+        ctx.set_timing(2)
+        ctx.cluster(x, eps, mp)
+        st_tests = ctx.stats()
+        ctx.set_timing(False)
     roofline = kernel_roofline(st, med, n, d, x.dtype.itemsize, a.workload)
     pair_ms = med["ms_core"] + med["ms_hook_kernel"] + med["ms_union_kernel"] + med["ms_border_kernel"]
 
@@ -679,7 +684,8 @@ def main() -> None:
             "note": "the reference's distance tests (3^D candidate cells per row) divided by the "
                     "summed time of the core, hook, union and border kernels, which skip tests "
                     "that bounding boxes decide"},
-        "core_tests": core_test_figures({**st, "core_tests": st_tests["core_tests"]}, med, d),
+        "core_tests": (core_test_figures({**st, "core_tests": st_tests["core_tests"]}, med, d)
+                       if a.core_tests else None),
         "stats": {k: int(st[k]) for k in ("occupied_cells", "core_points", "noise_points",
                                           "ambiguous_borders", "sum_candidates", "boxes",
                                           "neighbor_rows", "open_box_pairs")},
