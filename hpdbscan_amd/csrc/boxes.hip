@@ -127,12 +127,118 @@ __global__ __launch_bounds__(kThreads) void k_gather_boxes(
     }
 }
 
+// Bounding boxes only, the coordinates being in slot order already (the
+// sort carried them): k_gather_boxes' reduction over tiles of kBoundsRows
+// wave-segments per thread (a tile of kThreads * kBoundsRows slots per
+// block, all of a thread's slots loaded first), so that the LDS set-up,
+// the barriers and the stores of a block are shared by four times the
+// slots.  Rows of at most 8 bytes (LDS: 2 * tile * D ordered keys).
+constexpr int kBoundsRows = 4;
+constexpr int kBoundsTile = kThreads * kBoundsRows;
+
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_box_bounds(
+    const T* __restrict__ soa, int64_t ld, int64_t n, const int32_t* __restrict__ box_of_slot,
+    const int32_t* __restrict__ bstart, OrdKey<T>* __restrict__ bbox, OrdKey<T>* __restrict__ part) {
+    using U = OrdKey<T>;
+    static_assert(D * sizeof(T) <= 8, "rows of at most 8 bytes");
+    __shared__ U slo[kBoundsTile][D];
+    __shared__ U shi[kBoundsTile][D];
+    const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kBoundsTile;
+    for (int e = threadIdx.x; e < kBoundsTile; e += kThreads) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            slo[e][j] = static_cast<U>(~U(0));
+            shi[e][j] = U(0);
+        }
+    }
+    const int32_t b_first = box_of_slot[blk0];  // the block's first box
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    // segment k of wave w: slots blk0 + (k * kWavesPerBlock + w) * 64 + lane
+    int32_t b[kBoundsRows];
+    T x[kBoundsRows][D];
+#pragma unroll
+    for (int k = 0; k < kBoundsRows; ++k) {
+        const int64_t s = blk0 + (static_cast<int64_t>(k) * kWavesPerBlock + w) * kWave + lane_id();
+        const bool valid = s < n;
+        b[k] = valid ? box_of_slot[s] : -1 - static_cast<int32_t>(lane_id());
+#pragma unroll
+        for (int j = 0; j < D; ++j) x[k][j] = valid ? soa[static_cast<int64_t>(j) * ld + s] : T(0);
+    }
+    const int64_t blk1 = blk0 + kBoundsTile < n ? blk0 + kBoundsTile : n;
+    __syncthreads();
+    if (b_first == box_of_slot[blk1 - 1]) {
+        // the whole tile is one box (large boxes): each thread's slots,
+        // then one reduction per wave, then one LDS merge per wave
+        U lo[D], hi[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            lo[j] = static_cast<U>(~U(0));
+            hi[j] = U(0);
+        }
+#pragma unroll
+        for (int k = 0; k < kBoundsRows; ++k) {
+            if (b[k] < 0) continue;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const U key = ord_key(x[k][j]);
+                lo[j] = key < lo[j] ? key : lo[j];
+                hi[j] = key > hi[j] ? key : hi[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const U a = static_cast<U>(__shfl_xor(lo[j], o, 64));
+                const U z = static_cast<U>(__shfl_xor(hi[j], o, 64));
+                lo[j] = a < lo[j] ? a : lo[j];
+                hi[j] = z > hi[j] ? z : hi[j];
+            }
+            if (lane_id() == 0) {
+                atomicMin(&slo[0][j], lo[j]);
+                atomicMax(&shi[0][j], hi[j]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kBoundsRows; ++k) {
+            // the head lane of each run of the segment adds the run's extremes
+            const bool head = b[k] >= 0 && run_head(b[k]);
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const U key = ord_key(x[k][j]);
+                const U lo = run_suffix_min<U>(key, b[k]);
+                const U hi = run_suffix_max<U>(key, b[k]);
+                if (head) {
+                    atomicMin(&slo[b[k] - b_first][j], lo);
+                    atomicMax(&shi[b[k] - b_first][j], hi);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t nruns = box_of_slot[blk1 - 1] - b_first + 1;
+    for (int t = threadIdx.x; t < nruns; t += kThreads) {
+        const int32_t bb = b_first + t;
+        const bool starts = bstart[bb] >= blk0;
+        const bool ends = bstart[bb + 1] <= blk1;
+        U* dst = starts && ends ? &bbox[static_cast<int64_t>(bb) * D * 2]
+                                : &part[(2 * static_cast<int64_t>(blockIdx.x) + (starts ? 1 : 0)) * D * 2];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            dst[2 * j] = slo[t][j];
+            dst[2 * j + 1] = shi[t][j];
+        }
+    }
+}
+
 // Bounding boxes of the boxes that span blocks of k_gather_boxes: one wave
 // per block; the wave of the block where such a box starts combines the
 // block's share (its last run) with the shares of the following blocks (their
 // first runs) up to the box's last block.
 template <typename T, int D>
-__global__ __launch_bounds__(kThreads) void k_box_span(int64_t n, int64_t nblocks,
+__global__ __launch_bounds__(kThreads) void k_box_span(int64_t n, int64_t nblocks, int64_t tile,
                                                        const int32_t* __restrict__ box_of_slot,
                                                        const int32_t* __restrict__ bstart,
                                                        const OrdKey<T>* __restrict__ part,
@@ -140,12 +246,12 @@ __global__ __launch_bounds__(kThreads) void k_box_span(int64_t n, int64_t nblock
     using U = OrdKey<T>;
     const int64_t blk = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (blk >= nblocks) return;
-    const int64_t blk0 = blk * kThreads;
-    const int64_t blk1 = blk0 + kThreads < n ? blk0 + kThreads : n;
+    const int64_t blk0 = blk * tile;
+    const int64_t blk1 = blk0 + tile < n ? blk0 + tile : n;
     const int32_t b = box_of_slot[blk1 - 1];
     const int64_t b0 = bstart[b], b1 = bstart[b + 1];
     if (b0 < blk0 || b1 <= blk1) return;  // not a box that starts here and goes on
-    const int64_t last = (b1 - 1) / kThreads;
+    const int64_t last = (b1 - 1) / tile;
     U lo[D], hi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -1036,12 +1142,23 @@ void launch_gather_boxes(const T* pts, int64_t n, const uint32_t* inrow_of_slot,
                          void* part, hipStream_t s) {
     using U = OrdKey<T>;
     if (n <= 0) return;
-    const int64_t nb = (n + kThreads - 1) / kThreads;
-    hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, s,
-                       pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, static_cast<U*>(bbox),
-                       static_cast<U*>(part));
+    int64_t tile = kThreads;
+    if constexpr (D * sizeof(T) <= 8) {
+        if (!pts) tile = kBoundsTile;  // coordinates in place: bounding boxes only
+    }
+    const int64_t nb = (n + tile - 1) / tile;
+    if (tile == kThreads) {
+        hipLaunchKernelGGL((k_gather_boxes<T, D>), dim3(static_cast<unsigned>(nb)), dim3(kThreads),
+                           0, s, pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart,
+                           static_cast<U*>(bbox), static_cast<U*>(part));
+    } else {
+        if constexpr (D * sizeof(T) <= 8)
+            hipLaunchKernelGGL((k_box_bounds<T, D>), dim3(static_cast<unsigned>(nb)),
+                               dim3(kThreads), 0, s, soa, ld, n, box_of_slot, bstart,
+                               static_cast<U*>(bbox), static_cast<U*>(part));
+    }
     hipLaunchKernelGGL((k_box_span<T, D>), dim3(blocks_for(nb, kWavesPerBlock)), dim3(kThreads), 0,
-                       s, n, nb, box_of_slot, bstart, static_cast<const U*>(part),
+                       s, n, nb, tile, box_of_slot, bstart, static_cast<const U*>(part),
                        static_cast<U*>(bbox));
     HPD_HIP(hipGetLastError());
 }
