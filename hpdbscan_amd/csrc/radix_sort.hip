@@ -23,11 +23,17 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kIters = 16;
 constexpr int kTile = kThreads * kIters;  // 4096
+// passes that carry rows stage them in LDS too: half the tile keeps two
+// more blocks resident per CU
+constexpr int kItersRows = 8;
+constexpr int kTileMin = kThreads * kItersRows;
 constexpr int kMaxBuckets = 512;          // 9-bit digits
 
-__host__ __device__ inline int64_t tiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
+__host__ __device__ inline int64_t tiles_of(int64_t n, int64_t tile = kTile) {
+    return (n + tile - 1) / tile;
+}
 
-template <typename K, int kBits>
+template <typename K, int kBits, int kIt = kIters>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ keys, int64_t n,
                                                          int shift, uint32_t* __restrict__ hist,
                                                          int64_t ntiles,
@@ -40,9 +46,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const K* __restrict__ k
     __syncthreads();
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + w * (kTile / 4);
+    constexpr int tile = kThreads * kIt;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * tile + w * (tile / 4);
 #pragma unroll 4
-    for (int it = 0; it < kIters; ++it) {
+    for (int it = 0; it < kIt; ++it) {
         const int64_t idx = base + it * 64 + l;
         if (idx < n) {
             const uint32_t dgt = static_cast<uint32_t>((keys[idx] >> shift) & kMask);
@@ -86,7 +93,7 @@ struct MoveRows {
     }
 };
 
-template <typename K, int kBits, typename Rows = NoRows>
+template <typename K, int kBits, typename Rows = NoRows, int kIt = kIters>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             K* __restrict__ kout,
@@ -104,9 +111,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     __shared__ uint32_t gbase[B];
     __shared__ uint32_t tbase[B];  // digit's first place in the staged tile
     __shared__ uint32_t wsum[kThreads / 64];
-    __shared__ K skey[kTile];
-    __shared__ uint32_t sval[kTile];
-    __shared__ typename Rows::W srow[Rows::kOn ? kTile : 1];
+    constexpr int tile = kThreads * kIt;
+    __shared__ K skey[tile];
+    __shared__ uint32_t sval[tile];
+    __shared__ typename Rows::W srow[Rows::kOn ? tile : 1];
     const int w = threadIdx.x >> 6;
     const int l = threadIdx.x & 63;
     for (int dg = threadIdx.x; dg < B; dg += kThreads) {
@@ -116,15 +124,15 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     }
     __syncthreads();
 
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + w * (kTile / 4);
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * tile + w * (tile / 4);
     const uint64_t lt = lanemask_lt();
-    K key[kIters];
-    uint32_t val[kIters];
-    uint32_t rank[kIters];
-    uint32_t dgt[kIters];
-    typename Rows::W row[Rows::kOn ? kIters : 1];
+    K key[kIt];
+    uint32_t val[kIt];
+    uint32_t rank[kIt];
+    uint32_t dgt[kIt];
+    typename Rows::W row[Rows::kOn ? kIt : 1];
 #pragma unroll
-    for (int it = 0; it < kIters; ++it) {
+    for (int it = 0; it < kIt; ++it) {
         const int64_t idx = base + it * 64 + l;
         const bool valid = idx < n;
         const K k = valid ? kin[idx] : K(0);
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     // stage the tile in digit order (stable: waves, iterations and lanes are
     // in input order) ...
 #pragma unroll
-    for (int it = 0; it < kIters; ++it) {
+    for (int it = 0; it < kIt; ++it) {
         if (dgt[it] != 0xffffffffu) {
             const uint32_t at = tbase[dgt[it]] + cnt[w][dgt[it]] + rank[it];
             skey[at] = key[it];
@@ -196,10 +204,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const K* __restrict_
     __syncthreads();
     // ... and write it out: consecutive threads write consecutive places of
     // one digit's run
-    const int64_t left = n - static_cast<int64_t>(blockIdx.x) * kTile;
-    const int valid_in_tile = left < kTile ? static_cast<int>(left) : kTile;
+    const int64_t left = n - static_cast<int64_t>(blockIdx.x) * tile;
+    const int valid_in_tile = left < tile ? static_cast<int>(left) : tile;
 #pragma unroll
-    for (int it = 0; it < kIters; ++it) {
+    for (int it = 0; it < kIt; ++it) {
         const int at = it * kThreads + threadIdx.x;
         if (at < valid_in_tile) {
             const K k = skey[at];
@@ -266,7 +274,7 @@ template <typename K, int kBits, typename T, int D>
 bool sort_passes_rows(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, int64_t n,
                       int bits, uint32_t* hist, uint32_t* tmp, const T* pts, T* rows_a,
                       T* rows_b, T* soa, int64_t ld, hipStream_t s) {
-    const int64_t nt = tiles_of(n);
+    const int64_t nt = tiles_of(n, kTileMin);
     const int64_t nh = static_cast<int64_t>(1 << kBits) * nt;
     K* kin = keys_a;
     uint32_t* vin = vals_a;
@@ -280,10 +288,10 @@ bool sort_passes_rows(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
         using W = typename MoveRows<T, D>::W;
         MoveRows<T, D> mv{reinterpret_cast<const W*>(rin), reinterpret_cast<W*>(rnext), soa,
                           last ? ld : 0};
-        hipLaunchKernelGGL((k_radix_hist<K, kBits>), dim3(static_cast<unsigned>(nt)),
+        hipLaunchKernelGGL((k_radix_hist<K, kBits, kItersRows>), dim3(static_cast<unsigned>(nt)),
                            dim3(kThreads), 0, s, kin, n, shift, hist, nt, nullptr);
         exclusive_scan_u32(hist, hist, nh, tmp, nullptr, s);
-        hipLaunchKernelGGL((k_radix_scatter<K, kBits, MoveRows<T, D>>),
+        hipLaunchKernelGGL((k_radix_scatter<K, kBits, MoveRows<T, D>, kItersRows>),
                            dim3(static_cast<unsigned>(nt)), dim3(kThreads), 0, s, kin, vin, kout,
                            vout, n, shift, hist, nt, nullptr, mv);
         HPD_HIP(hipGetLastError());
@@ -309,7 +317,7 @@ int radix_pass_count(int bits) {
 }
 
 size_t radix_hist_words(int64_t n) {
-    return static_cast<size_t>(kMaxBuckets) * static_cast<size_t>(tiles_of(n > 0 ? n : 1));
+    return static_cast<size_t>(kMaxBuckets) * static_cast<size_t>(tiles_of(n > 0 ? n : 1, kTileMin));
 }
 
 // 9-bit digits where they save a pass (e.g. 25-27 key bits: 3 passes, not 4)
