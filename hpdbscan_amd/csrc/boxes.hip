@@ -302,14 +302,18 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
                                                         int32_t* __restrict__ box_of_slot,
                                                         const uint32_t* __restrict__ nbox_dev,
                                                         void* __restrict__ bbox, int bbox_keys,
-                                                        int key_bytes) {
+                                                        int key_bytes, int shift,
+                                                        uint32_t* __restrict__ cbflag) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i < n) {
         const uint32_t b = scan[i] + flags[i] - 1u;
         box_of_slot[i] = static_cast<int32_t>(b);
         if (flags[i]) {
             bstart[b] = static_cast<int32_t>(i);
-            bcell[b] = static_cast<int32_t>(cscan[i] + cflags[i] - 1u);  // occupied-cell index
+            if (cflags)
+                bcell[b] = static_cast<int32_t>(cscan[i] + cflags[i] - 1u);  // occupied-cell index
+            else  // the box starts a cell: the cells come from the boxes
+                cbflag[b] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
             // empty bounding box: min keys all ones, max keys zero
             if (bbox) {
                 const int64_t k0 = static_cast<int64_t>(b) * bbox_keys;
@@ -323,6 +327,29 @@ __global__ __launch_bounds__(kThreads) void k_box_table(const K* __restrict__ ke
         }
     }
     if (i == 0) bstart[*nbox_dev] = static_cast<int32_t>(n);
+}
+
+// The occupied cells from the boxes (a cell's first slot starts a box):
+// cell u = the u-th box with cbflag set; ucell / ustart as the cell runs
+// give them (ustart[m] = n), bcell per box.  The boxes walk a bounded grid.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_cells_of_boxes(
+    const K* __restrict__ keys, int64_t n, int shift, const int32_t* __restrict__ bstart,
+    Count nbox_c, const uint32_t* __restrict__ cbflag, const uint32_t* __restrict__ cbscan,
+    int32_t* __restrict__ bcell, K* __restrict__ ucell, int32_t* __restrict__ ustart,
+    const uint32_t* __restrict__ ncells_dev) {
+    const int64_t nbox = nbox_c.get();
+    for (int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; b < nbox;
+         b += static_cast<int64_t>(gridDim.x) * kThreads) {
+        const uint32_t f = cbflag[b], u = cbscan[b];
+        bcell[b] = static_cast<int32_t>(u + f - 1u);
+        if (f) {
+            const int32_t s0 = bstart[b];
+            ucell[u] = keys[s0] >> shift;
+            ustart[u] = s0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ustart[*ncells_dev] = static_cast<int32_t>(n);
 }
 
 // first box of every cell; cbox[m] = number of boxes
@@ -1172,14 +1199,33 @@ template <typename K>
 void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const uint32_t* cscan,
                       uint32_t* flags, uint32_t* scan, uint32_t* scan_tmp, int32_t* bstart,
                       int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev, void* bbox,
-                      int bbox_keys, int key_bytes, hipStream_t s) {
+                      int bbox_keys, int key_bytes, hipStream_t s, int shift, uint32_t* cbflag) {
+    if (!cflags && !cbflag)
+        throw Error(HPD_EINVAL, "launch_box_table: cell flags or the per-box flag buffer needed");
     hipLaunchKernelGGL(k_box_flags<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
                        flags);
     exclusive_scan_u32(flags, scan, n, scan_tmp, nbox_dev, s);
     hipLaunchKernelGGL(k_box_table<K>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, keys, n,
                        static_cast<const uint32_t*>(flags), static_cast<const uint32_t*>(scan),
                        cflags, cscan, bstart, bcell, box_of_slot,
-                       static_cast<const uint32_t*>(nbox_dev), bbox, bbox_keys, key_bytes);
+                       static_cast<const uint32_t*>(nbox_dev), bbox, bbox_keys, key_bytes, shift,
+                       cbflag);
+    HPD_HIP(hipGetLastError());
+}
+
+template <typename K>
+void launch_cells_of_boxes(const K* keys, int64_t n, int shift, const int32_t* bstart,
+                           uint32_t* nbox_dev, uint32_t* cbflag, uint32_t* cbscan,
+                           uint32_t* scan_tmp, int32_t* bcell, K* ucell, int32_t* ustart,
+                           uint32_t* ncells_dev, hipStream_t s) {
+    const Count nb{n, nbox_dev};
+    exclusive_scan_u32_dev(cbflag, cbscan, nb, scan_tmp, s);
+    copy_at_count(cbscan, nbox_dev, ncells_dev, s);
+    const int64_t want = blocks_for(n, kThreads);
+    hipLaunchKernelGGL(k_cells_of_boxes<K>, dim3(static_cast<unsigned>(want < 4096 ? want : 4096)),
+                       dim3(kThreads), 0, s, keys, n, shift, bstart, nb,
+                       static_cast<const uint32_t*>(cbflag), static_cast<const uint32_t*>(cbscan),
+                       bcell, ucell, ustart, static_cast<const uint32_t*>(ncells_dev));
     HPD_HIP(hipGetLastError());
 }
 
@@ -1449,10 +1495,16 @@ HPD_INST_BOX_ALL_D(double)
 template void launch_box_table<uint32_t>(const uint32_t*, int64_t, const uint32_t*,
                                          const uint32_t*, uint32_t*, uint32_t*, uint32_t*,
                                          int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
-                                         hipStream_t);
+                                         hipStream_t, int, uint32_t*);
 template void launch_box_table<uint64_t>(const uint64_t*, int64_t, const uint32_t*,
                                          const uint32_t*, uint32_t*, uint32_t*, uint32_t*,
                                          int32_t*, int32_t*, int32_t*, uint32_t*, void*, int, int,
-                                         hipStream_t);
+                                         hipStream_t, int, uint32_t*);
+template void launch_cells_of_boxes<uint32_t>(const uint32_t*, int64_t, int, const int32_t*,
+                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, int32_t*,
+                                              uint32_t*, int32_t*, uint32_t*, hipStream_t);
+template void launch_cells_of_boxes<uint64_t>(const uint64_t*, int64_t, int, const int32_t*,
+                                              uint32_t*, uint32_t*, uint32_t*, uint32_t*, int32_t*,
+                                              uint64_t*, int32_t*, uint32_t*, hipStream_t);
 
 }  // namespace hpd

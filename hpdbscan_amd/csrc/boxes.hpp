@@ -43,15 +43,26 @@ size_t gather_part_bytes(int64_t n, int d, int value_bytes);
 
 // Boxes of keys sorted by (cell, sub-box digit): runs of equal keys.
 // cflags / cscan are the cell-run flags and their exclusive scan (they give
-// each box's occupied-cell index); flags and scan (n words each) and
-// scan_tmp are scratch; bstart: nbox + 1 entries; writes *nbox_dev; bbox
-// (when not null, bbox_keys keys of key_bytes each per box) is set to
-// empty boxes.
+// each box's occupied-cell index); without them (null) cbflag[b] is set to
+// whether box b starts a cell (key >> shift changes), for
+// launch_cells_of_boxes.  flags and scan (n words each) and scan_tmp are
+// scratch; bstart: nbox + 1 entries; writes *nbox_dev; bbox (when not null,
+// bbox_keys keys of key_bytes each per box) is set to empty boxes.
 template <typename K>
 void launch_box_table(const K* keys, int64_t n, const uint32_t* cflags, const uint32_t* cscan,
                       uint32_t* flags, uint32_t* scan, uint32_t* scan_tmp, int32_t* bstart,
                       int32_t* bcell, int32_t* box_of_slot, uint32_t* nbox_dev, void* bbox,
-                      int bbox_keys, int key_bytes, hipStream_t s);
+                      int bbox_keys, int key_bytes, hipStream_t s, int shift = 0,
+                      uint32_t* cbflag = nullptr);
+// The occupied cells from the boxes of launch_box_table (every cell starts
+// a box): ucell, ustart (ustart[m] = n) and *ncells_dev as launch_cells
+// gives them, and bcell per box, from one scan over the boxes instead of
+// a flag pass and a scan over all n slots.  cbscan: n + 1 words.
+template <typename K>
+void launch_cells_of_boxes(const K* keys, int64_t n, int shift, const int32_t* bstart,
+                           uint32_t* nbox_dev, uint32_t* cbflag, uint32_t* cbscan,
+                           uint32_t* scan_tmp, int32_t* bcell, K* ucell, int32_t* ustart,
+                           uint32_t* ncells_dev, hipStream_t s);
 void launch_cell_boxes(const int32_t* bcell, Count nbox, Count m, int32_t* cbox, hipStream_t s);
 
 // cnt[b] = 64-slot chunks of box b, scanned in place; *total_dev = chunks

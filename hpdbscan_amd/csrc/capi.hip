@@ -433,8 +433,9 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
     uint32_t* vals_free = in_b ? vals_a : vals_b;
 
     // occupied cells (spatial_index.h:167-182); their order and the order of
-    // input rows inside them are the reference's sorted order
-    launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &counters[0], s);
+    // input rows inside them are the reference's sorted order (box mode:
+    // from the boxes, below)
+    if (!boxes) launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &counters[0], s);
 
     // box mode: slots are rows sorted by cell, sub-box, input row; the
     // reference's sorted positions are only materialised where needed
@@ -460,10 +461,15 @@ static void run_keyed(Context& ctx, const T* pts, int64_t n, const JobParams<T>*
         btight = A.take_n<uint8_t>(nn);
         bbox = A.take(2 * D * sizeof(T) * nn);
         void* bpart = A.take(gather_part_bytes(n, D, static_cast<int>(sizeof(T))));
-        // box flags and their scan in the sort's free buffers
-        launch_box_table<K>(keys, n, flags, flags + n, vals_free,
+        // box flags and their scan in the sort's free buffers; then the
+        // occupied cells from the boxes (per-box cell flags and their scan
+        // in the run-flag buffer)
+        launch_box_table<K>(keys, n, nullptr, nullptr, vals_free,
                             reinterpret_cast<uint32_t*>(keys_free), tmp, bstart, bcell, box_of_slot,
-                            &counters[3], nullptr, 2 * D, static_cast<int>(sizeof(T)), s);
+                            &counters[3], nullptr, 2 * D, static_cast<int>(sizeof(T)), s, sub_bits,
+                            flags);
+        launch_cells_of_boxes<K>(keys, n, sub_bits, bstart, &counters[3], flags, flags + nn + 1,
+                                 tmp, bcell, ucell, ustart, &counters[0], s);
         launch_gather_boxes<T, D>(rows_sorted ? nullptr : pts, n, inrow_of_slot, soa, ld,
                                   box_of_slot, bstart, bbox, bpart, s);
     } else if (!rows_sorted) {
