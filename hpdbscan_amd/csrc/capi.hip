@@ -924,10 +924,12 @@ static void static_body(Context& ctx, const T* pts, int64_t n, JobParams<T>* job
                             reinterpret_cast<const int*>(&sc[SS_BITS]), hist, tmp, s);
     K* keys = keys_a;
     uint32_t* inrow_of_slot = vals_a;
-    launch_cells<K>(keys, n, sub_bits, flags, tmp, ucell, ustart, &sc[SS_M], s);
-    launch_box_table<K>(keys, n, flags, flags + n, vals_b, reinterpret_cast<uint32_t*>(keys_b), tmp,
+    // the boxes, then the occupied cells from them
+    launch_box_table<K>(keys, n, nullptr, nullptr, vals_b, reinterpret_cast<uint32_t*>(keys_b), tmp,
                         bstart, bcell, box_of_slot, &sc[SS_NBOX], nullptr, 2 * D,
-                        static_cast<int>(sizeof(T)), s);
+                        static_cast<int>(sizeof(T)), s, sub_bits, flags);
+    launch_cells_of_boxes<K>(keys, n, sub_bits, bstart, &sc[SS_NBOX], flags, flags + nn + 1, tmp,
+                             bcell, ucell, ustart, &sc[SS_M], s);
     launch_gather_boxes<T, D>(pts, n, inrow_of_slot, soa, ld, box_of_slot, bstart, bbox, bpart, s);
 
     // neighbour rows and work lists, counted and laid out on the device
